@@ -1,0 +1,133 @@
+/*
+ * radik.h -- C-ABI of libradik_hip.so, the MI355X (gfx950) radix top-k library.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses;
+ * hipStream_t values are passed as void* (NULL = the default stream).
+ * Every function returns 0 on success or one of the RADIK_E* codes below, and
+ * leaves a message for radik_last_error() (per calling thread).
+ *
+ * The Julia shim (radik.jl_amd/julia/RadiK/src/RadiK.jl) binds these with
+ * ccall; the Python mirror (radik.jl_amd/radik) with ctypes.  Each entry cites
+ * the reference interface (WilliBee/RadiK.jl) it replaces.
+ */
+#ifndef RADIK_H
+#define RADIK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Flag bits: the Val switches of topk_radix_select! (src/algorithm.jl:170-175). */
+#define RADIK_LARGEST     0x01u /* Val{LARGEST}: top-k by largest value             */
+#define RADIK_ASCEND      0x02u /* Val{ASCEND}: output sorted ascending (= !rev)    */
+#define RADIK_WITHSCALE   0x04u /* Val{WITHSCALE}: select on x - first element      */
+#define RADIK_WITHIDXIN   0x08u /* Val{WITHIDXIN}: idx_out = idx_in[pos]            */
+#define RADIK_WITHPACKING 0x10u /* Val{WITHPACKING}: vector loads (vload_bytes)    */
+
+/* Status codes. */
+#define RADIK_OK            0
+#define RADIK_EINVAL        1 /* bad argument (k < 0, null pointer, bad flags)       */
+#define RADIK_ESIZE         2 /* output size mismatch (algorithm.jl:498-501)        */
+#define RADIK_EINDICES      3 /* indices length != data length (algorithm.jl:507-509) */
+#define RADIK_EWORKSPACE    4 /* workspace too small: max(task_lens) > n or T > num_tasks */
+#define RADIK_EHIP          5 /* HIP runtime error                                   */
+#define RADIK_ENOMEM        6 /* device or pinned allocation failed                  */
+#define RADIK_EADMIT        7 /* serving: request exceeds the workspace budget       */
+#define RADIK_ENOTREADY     8 /* serving: result not complete yet                    */
+
+typedef struct radik_ws radik_ws;
+typedef struct radik_plan radik_plan;
+typedef struct radik_server radik_server;
+
+/* Library identity and errors. */
+const char *radik_version(void);
+const char *radik_last_error(void);
+
+/* RadiKWorkspace(backend, n, num_tasks, ::Type{I}=Int32)   src/algorithm.jl:58-83.
+ * n: longest task the workspace serves; num_tasks: most tasks per call;
+ * counter_bytes: 4 (Int32) or 8 (Int64), sizeof(I).  Device buffers are
+ * allocated on the current device; the spill buffer grows on first use. */
+int radik_ws_create(radik_ws **ws, int64_t n, int64_t num_tasks, int counter_bytes, void *stream);
+int radik_ws_destroy(radik_ws *ws);
+
+/* topk_radix_select!(val_out, idx_out, ws, val_in, idx_in, task_lens, k,
+ *     Val(LARGEST), Val(ASCEND), Val(WITHSCALE), Val(WITHIDXIN),
+ *     Val(WITHPACKING), Val(VLOAD_SIZE_BYTES))      src/algorithm.jl:162-368.
+ * val_in: concatenated tasks (device, sum(task_lens) floats); idx_in: same
+ * length, int32 or int64 (idx_bytes) when RADIK_WITHIDXIN, else ignored;
+ * task_lens: HOST array of num_tasks lengths; val_out/idx_out: device [k,
+ * num_tasks] column-major.  Outputs per task: min(k, len) sorted results,
+ * then 0.0f / -1 padding.  Synchronises the stream before returning, like
+ * the reference (algorithm.jl:353). */
+int radik_topk_radix_select(radik_ws *ws, float *val_out, void *idx_out, int idx_bytes,
+                            const float *val_in, const void *idx_in, const int64_t *task_lens,
+                            int64_t num_tasks, int64_t k, uint32_t flags, int vload_bytes, void *stream);
+
+/* topk!(val_out, idx_out, data, task_lens, k; indices, largest, rev)
+ * src/algorithm.jl:477-522: validates output sizes (out_rows x out_cols must
+ * be k x num_tasks) and the indices length (n_indices must be 0 or
+ * sum(task_lens)), allocates a workspace for this call (as the reference
+ * does, algorithm.jl:503), runs with WITHPACKING and 32-byte loads. */
+int radik_topk(float *val_out, void *idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
+               const float *val_in, const void *indices, int64_t n_indices, const int64_t *task_lens,
+               int64_t num_tasks, int64_t k, int largest, int rev, void *stream);
+
+/* Persistent batched form (no reference counterpart: the reference launches
+ * one grid per pass, algorithm.jl:221-334).  A plan fixes task lengths, a k
+ * per task and flags; radik_plan_execute enqueues the whole pass sequence
+ * asynchronously, replaying a captured hipGraph.  Task t reads val_in at the
+ * sum of earlier lengths and writes k_per_task[t] outputs to val_out/idx_out
+ * at the sum of earlier k's (equal k gives the [k, T] layout). */
+int radik_plan_create(radik_plan **plan, const int64_t *task_lens, const int64_t *k_per_task,
+                      int64_t num_tasks, uint32_t flags, int idx_bytes, void *stream);
+int radik_plan_execute(radik_plan *plan, float *val_out, void *idx_out, const float *val_in,
+                       const void *idx_in, void *stream);
+/* use_graph = 0 launches the kernels eagerly (same results). */
+int radik_plan_set_graph(radik_plan *plan, int use_graph);
+/* Per-kernel timing: when enabled, execute runs eagerly and records HIP events
+ * around each kernel on the plan's stream; radik_plan_timing returns the
+ * accumulated milliseconds per phase (hist, select, compact, refine, sort,
+ * small) and the number of executes timed. */
+int radik_plan_set_timing(radik_plan *plan, int enable);
+int radik_plan_timing(radik_plan *plan, double *ms6, int64_t *count);
+int radik_plan_destroy(radik_plan *plan);
+
+/* Serving engine (no reference counterpart): continuous batching of top-k
+ * requests against a device workspace budget.  Requests are row-major
+ * matrices on the device (rows x cols floats); each row is one task.
+ * submit never blocks on the GPU; poll reports completion; results land in
+ * the caller's val_out/idx_out (rows x k, row-major, int32 indices,
+ * 1-based column positions). */
+int radik_server_create(radik_server **srv, int64_t budget_bytes, int max_inflight, uint32_t flags);
+int radik_server_submit(radik_server *srv, const float *in, int64_t rows, int64_t cols, int64_t k,
+                        float *val_out, int32_t *idx_out, int64_t *ticket);
+/* Returns RADIK_OK when the ticket's results are complete (latency_ms: host
+ * submit -> completion event), RADIK_ENOTREADY otherwise. */
+int radik_server_poll(radik_server *srv, int64_t ticket, double *latency_ms);
+int radik_server_wait(radik_server *srv, int64_t ticket, double *latency_ms);
+int radik_server_drain(radik_server *srv);
+int radik_server_stats(radik_server *srv, int64_t *launches, int64_t *rows_done, int64_t *max_coalesced);
+int radik_server_destroy(radik_server *srv);
+
+/* Pass-level diagnostics for parity tests (no reference counterpart; they
+ * expose what the reference's kernel unit tests inspect, test/test_count_bin.jl
+ * and test/test_select_bin.jl).  Runs pass 1 (histogram of the rank-key digit
+ * [31:20]) and the bin selection for the tasks; hist_out receives num_tasks x
+ * 4096 uint32 counts (device), sel_out num_tasks x 4 uint32 (device): winner
+ * bound, candidate bin (0xFFFFFFFF: none), residual k, candidate count. */
+int radik_diag_pass1(const float *val_in, const int64_t *task_lens, int64_t num_tasks, int64_t k,
+                     uint32_t flags, uint32_t *hist_out, uint32_t *sel_out, void *stream);
+/* Wave64 inclusive scan (DPP) of n uint32 values per 64-lane wave and the
+ * 256-thread block scan built on it, as the selection kernels use them
+ * (counterpart of @warpreduce, src/utils.jl:84-100): out_wave[i] = inclusive
+ * scan within i's wave, out_block[i] = inclusive scan within i's block of 256. */
+int radik_diag_scan(const uint32_t *in, uint32_t *out_wave, uint32_t *out_block, int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RADIK_H */

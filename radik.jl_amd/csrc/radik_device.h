@@ -1,0 +1,158 @@
+// radik_device.h -- layouts and device helpers shared by the gfx950 kernels
+// and the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace radik {
+
+// Per-task flag bits carried in TaskDesc::flags (the launch-level C-ABI flags
+// plus the index width).
+constexpr uint32_t F_LARGEST = 0x01u;
+constexpr uint32_t F_ASCEND = 0x02u;
+constexpr uint32_t F_WITHSCALE = 0x04u;
+constexpr uint32_t F_WITHIDXIN = 0x08u;
+constexpr uint32_t F_IDX64 = 0x100u;
+
+constexpr uint32_t NO_BIN = 0xFFFFFFFFu;
+constexpr int HIST_BINS = 4096;  // first digit: rank-key bits [31:20]
+constexpr int BLOCK = 256;       // threads of every kernel in this library
+
+// One task of a launch.  Pointers address the task's own first element, so a
+// launch may mix tasks from different buffers (the serving engine does).
+struct TaskDesc {
+  const float* in;      // len values
+  const void* idx_in;   // len indices (int32/int64) when F_WITHIDXIN
+  float* val_out;       // k values
+  void* idx_out;        // k indices (int32/int64)
+  int64_t len;          // elements (< 2^32)
+  int64_t base;         // start in the launch's concatenated element space;
+                        // also the task's region of the candidate buffer
+  int32_t k;            // requested k (>= 0)
+  uint32_t flags;       // F_* bits
+  int64_t reserved;
+};
+static_assert(sizeof(TaskDesc) == 64, "TaskDesc layout");
+
+// Per-task state between the passes, written by k_select.
+struct TaskCtl {
+  uint32_t b_win;     // first-digit values < b_win are winners
+  uint32_t b_cand;    // first-digit value of the candidates, NO_BIN if none
+  uint32_t krem;      // candidates still to take (0: none)
+  uint32_t ncand;     // candidates expected (= count of bin b_cand)
+  uint32_t nwin;      // winners expected (= m - krem)
+  uint32_t m;         // min(k, len): valid outputs
+  uint32_t win_ctr;   // slot counter for winners (compaction)
+  uint32_t cand_ctr;  // slot counter for candidates (compaction)
+  uint32_t pad[8];
+};
+static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
+
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+
+// Order-preserving key of a float: unsigned order of key == numeric order,
+// with -0 < +0 and NaNs outside +-Inf (RadiK.jl src/utils.jl:29-33).
+__device__ __forceinline__ uint32_t fkey(float x) {
+  uint32_t u = __float_as_uint(x);
+  return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+}
+
+__device__ __forceinline__ float fkey_inv(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k ^ 0x80000000u) : ~k;
+  return __uint_as_float(u);
+}
+
+// Scaled value: x - s for WITHSCALE tasks (s = 0 otherwise, which is exact
+// for every non-NaN x); NaN keeps its own bits.
+__device__ __forceinline__ float scaled(float x, float s) {
+  float d = x - s;
+  return (x == x) ? d : x;
+}
+
+// Rank key: smaller is better.  xm = 0xFFFFFFFF for LARGEST, 0 otherwise.
+__device__ __forceinline__ uint32_t rank_key(float x, float s, uint32_t xm) {
+  return fkey(scaled(x, s)) ^ xm;
+}
+
+// The WITHSCALE scaler of a task: its first element, 0 if NaN/Inf
+// (src/utils.jl:62-66); 0 when the task does not scale.
+__device__ __forceinline__ float task_scaler(const TaskDesc& d) {
+  if (!(d.flags & F_WITHSCALE) || d.len <= 0) return 0.0f;
+  float s = d.in[0];
+  return (isnan(s) || isinf(s)) ? 0.0f : s;
+}
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Lanes of mask below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) {
+  return __builtin_amdgcn_ballot_w64(p);
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave with DPP row shifts and
+// the GFX9 row broadcasts: rows of 16 first, then row 0 -> 1, 2 -> 3, and
+// lane 31 -> rows 2 and 3.  Lanes of disabled rows or without a source read 0.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+
+// Inclusive scan over a block of BLOCK threads (4 waves): wave scan, then the
+// wave totals through LDS.  wsum: LDS scratch of BLOCK/64 words.  Ends with
+// the block synchronised.
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t incl = wave_incl_scan(v);
+  if ((tid & 63u) == 63u) wsum[tid >> 6] = incl;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (uint32_t w = 0; w < (tid >> 6); ++w) pre += wsum[w];
+  __syncthreads();
+  return incl + pre;
+}
+
+// Finds the bin b with before(b) < need <= before(b) + count(b), where each
+// thread holds PER consecutive bins (thread i: bins i*PER .. i*PER+PER-1) and
+// before(b) sums all lower bins.  Requires 1 <= need <= total.  res (LDS, 3
+// words) receives b, before(b), count(b); the block is synchronised on return.
+template <int PER>
+__device__ __forceinline__ void block_find_bin(const uint32_t (&c)[PER], uint32_t need,
+                                               uint32_t* wsum, uint32_t* res) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s += c[j];
+  const uint32_t incl = block_incl_scan(s, wsum);
+  const uint32_t excl = incl - s;
+  if (excl < need && need <= incl) {
+    uint32_t before = excl;
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (!done && before + c[j] >= need) {
+        res[0] = threadIdx.x * PER + j;
+        res[1] = before;
+        res[2] = c[j];
+        done = true;
+      }
+      before += c[j];
+    }
+  }
+  __syncthreads();
+}
+
+#endif  // device
+
+}  // namespace radik

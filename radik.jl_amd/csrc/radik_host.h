@@ -1,0 +1,88 @@
+// radik_host.h -- host-side pieces shared by the C-ABI translation units:
+// error reporting, device facts, the launch pipeline and its buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "radik_device.h"
+
+namespace radik {
+
+// Thread-local error slot behind radik_last_error().
+int set_error(int code, const char* fmt, ...);
+void clear_error();
+
+#define RADIK_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return ::radik::set_error(5, "%s failed: %s", #call, hipGetErrorString(e_));  \
+  } while (0)
+
+// Facts of the current device, queried once per device.
+struct DevFacts {
+  int device = -1;
+  int cus = 256;
+  size_t lds_per_block = 65536;
+  uint32_t small_cap = 2048;   // longest task routed to k_small
+  uint32_t refine_cap = 4096;  // candidates kept in LDS by k_refine
+  uint32_t sort_cap = 4096;    // results sorted in LDS by k_sort
+};
+int dev_facts(DevFacts** out);
+
+// Launch-level options.
+struct LaunchOpts {
+  int unr = 2;            // float4 loads in flight per thread (vload bytes / 16)
+  int blocks_per_cu = 8;  // streaming grid: blocks per CU
+};
+
+// Device + pinned buffers of one pipeline instance.  Grow-only; the
+// histogram is kept all-zero between launches (k_select clears what it reads).
+struct Pipeline {
+  // host copies (pinned)
+  TaskDesc* h_large = nullptr;
+  int64_t* h_bases = nullptr;
+  TaskDesc* h_small = nullptr;
+  // device
+  TaskDesc* d_large = nullptr;
+  int64_t* d_bases = nullptr;
+  TaskDesc* d_small = nullptr;
+  TaskCtl* d_ctl = nullptr;
+  uint32_t* d_hist = nullptr;
+  uint64_t* d_cand = nullptr;
+  size_t cap_large = 0, cap_small = 0, cap_hist = 0, cap_cand = 0;
+  // current content
+  int64_t n_large = 0, n_small = 0, total = 0;
+  hipEvent_t upload_done = nullptr;  // pinned buffers reusable after this
+  ~Pipeline();
+  int reserve(int64_t n_large, int64_t n_small, int64_t total_large, hipStream_t s);
+};
+
+// Splits tasks into short ones (k_small) and long ones (streaming passes),
+// fills the pinned descriptor arrays and their prefix sums.
+struct TaskSpec {
+  const float* in;
+  const void* idx_in;
+  float* val_out;
+  void* idx_out;
+  int64_t len;
+  int64_t k;
+  uint32_t flags;
+};
+int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s);
+
+// Phase timing hooks (events recorded between kernels when non-null).
+struct PhaseTimer {
+  hipEvent_t ev[7] = {};
+  bool on = false;
+};
+
+// Copies the staged descriptors to the device and enqueues every kernel.
+int upload(Pipeline& p, hipStream_t s);
+int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm);
+
+}  // namespace radik

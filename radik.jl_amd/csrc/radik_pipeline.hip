@@ -1,0 +1,305 @@
+// radik_pipeline.hip -- host orchestration of the kernel sequence.  The
+// kernels are compiled in this translation unit so every launch resolves in
+// one code object.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "radik_host.h"
+#include "radik_kernels.hip"
+
+namespace radik {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+const char* last_error() { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Device facts
+// ---------------------------------------------------------------------------
+static std::mutex g_facts_mu;
+static DevFacts g_facts[64];
+
+int dev_facts(DevFacts** out) {
+  int dev = 0;
+  RADIK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_error(5, "device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_facts_mu);
+  DevFacts& f = g_facts[dev];
+  if (f.device != dev) {
+    int cus = 0, lds = 0;
+    RADIK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RADIK_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    f.cus = cus > 0 ? cus : 256;
+    // LDS budget per block: the device's limit, at most 64 KiB until larger
+    // dynamic allocations are enabled per kernel below.
+    size_t budget = (size_t)(lds > 0 ? lds : 65536);
+    if (budget > 65536) {
+      // Opt the per-task kernels into the larger allocation.
+      const int want = (int)budget;
+      bool ok = hipFuncSetAttribute((const void*)k_refine, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    want - 20 * 1024) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    want - 20 * 1024) == hipSuccess;
+      if (!ok) {
+        (void)hipGetLastError();
+        budget = 65536;
+      }
+    }
+    f.lds_per_block = budget;
+    const size_t stat = 20 * 1024;  // static LDS of k_refine / k_small (histogram + scratch)
+    f.refine_cap = (uint32_t)((budget - stat) / 8);
+    f.sort_cap = (uint32_t)(budget / 8);
+    f.small_cap = (uint32_t)((budget - stat) / 16);
+    f.device = dev;
+  }
+  *out = &f;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Buffers
+// ---------------------------------------------------------------------------
+Pipeline::~Pipeline() {
+  if (h_large) (void)hipHostFree(h_large);
+  if (h_bases) (void)hipHostFree(h_bases);
+  if (h_small) (void)hipHostFree(h_small);
+  if (d_large) (void)hipFree(d_large);
+  if (d_bases) (void)hipFree(d_bases);
+  if (d_small) (void)hipFree(d_small);
+  if (d_ctl) (void)hipFree(d_ctl);
+  if (d_hist) (void)hipFree(d_hist);
+  if (d_cand) (void)hipFree(d_cand);
+  if (upload_done) (void)hipEventDestroy(upload_done);
+}
+
+static size_t grow(size_t have, size_t need) {
+  size_t c = have ? have : 16;
+  while (c < need) c *= 2;
+  return c;
+}
+
+int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
+  if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
+  // The pinned arrays may still feed an in-flight copy.
+  RADIK_HIP(hipEventSynchronize(upload_done));
+  if ((size_t)nl > cap_large) {
+    // Device buffers may be in use by queued work on s.
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = grow(cap_large, (size_t)nl);
+    if (h_large) RADIK_HIP(hipHostFree(h_large));
+    if (h_bases) RADIK_HIP(hipHostFree(h_bases));
+    if (d_large) RADIK_HIP(hipFree(d_large));
+    if (d_bases) RADIK_HIP(hipFree(d_bases));
+    if (d_ctl) RADIK_HIP(hipFree(d_ctl));
+    h_large = nullptr; h_bases = nullptr; d_large = nullptr; d_bases = nullptr; d_ctl = nullptr;
+    if (hipHostMalloc((void**)&h_large, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_bases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_large, c * sizeof(TaskDesc)) != hipSuccess ||
+        hipMalloc((void**)&d_bases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc((void**)&d_ctl, c * sizeof(TaskCtl)) != hipSuccess) {
+      cap_large = 0;
+      return set_error(6, "allocation of %zu task descriptors failed", c);
+    }
+    cap_large = c;
+  }
+  if ((size_t)nl > cap_hist) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = grow(cap_hist, (size_t)nl);
+    if (d_hist) RADIK_HIP(hipFree(d_hist));
+    d_hist = nullptr;
+    if (hipMalloc((void**)&d_hist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess) {
+      cap_hist = 0;
+      return set_error(6, "allocation of %zu task histograms failed", c);
+    }
+    RADIK_HIP(hipMemset(d_hist, 0, c * HIST_BINS * sizeof(uint32_t)));
+    cap_hist = c;
+  }
+  if ((size_t)ns > cap_small) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = grow(cap_small, (size_t)ns);
+    if (h_small) RADIK_HIP(hipHostFree(h_small));
+    if (d_small) RADIK_HIP(hipFree(d_small));
+    h_small = nullptr; d_small = nullptr;
+    if (hipHostMalloc((void**)&h_small, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_small, c * sizeof(TaskDesc)) != hipSuccess) {
+      cap_small = 0;
+      return set_error(6, "allocation of %zu short-task descriptors failed", c);
+    }
+    cap_small = c;
+  }
+  if ((size_t)tot > cap_cand) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = (size_t)tot;
+    if (d_cand) RADIK_HIP(hipFree(d_cand));
+    d_cand = nullptr;
+    if (hipMalloc((void**)&d_cand, c * sizeof(uint64_t)) != hipSuccess) {
+      cap_cand = 0;
+      return set_error(6, "allocation of a %zu-entry candidate spill buffer failed", c);
+    }
+    cap_cand = c;
+  }
+  return 0;
+}
+
+int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s) {
+  int64_t nl = 0, ns = 0, tot = 0;
+  for (int64_t t = 0; t < n; ++t) {
+    if (tasks[t].len <= (int64_t)f.small_cap) ++ns;
+    else { ++nl; tot += tasks[t].len; }
+  }
+  int rc = p.reserve(nl, ns, tot, s);
+  if (rc) return rc;
+  int64_t il = 0, is = 0, base = 0;
+  for (int64_t t = 0; t < n; ++t) {
+    const TaskSpec& ts = tasks[t];
+    TaskDesc d;
+    d.in = ts.in;
+    d.idx_in = ts.idx_in;
+    d.val_out = ts.val_out;
+    d.idx_out = ts.idx_out;
+    d.len = ts.len;
+    d.k = (int32_t)ts.k;
+    d.flags = ts.flags;
+    if (ts.len <= (int64_t)f.small_cap) {
+      d.base = 0;
+      p.h_small[is++] = d;
+    } else {
+      d.base = base;
+      p.h_bases[il] = base;
+      p.h_large[il++] = d;
+      base += ts.len;
+    }
+  }
+  p.h_bases[il] = base;
+  p.n_large = nl;
+  p.n_small = ns;
+  p.total = tot;
+  return 0;
+}
+
+int upload(Pipeline& p, hipStream_t s) {
+  if (p.n_large) {
+    RADIK_HIP(hipMemcpyAsync(p.d_large, p.h_large, p.n_large * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
+    RADIK_HIP(hipMemcpyAsync(p.d_bases, p.h_bases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  }
+  if (p.n_small)
+    RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
+  RADIK_HIP(hipEventRecord(p.upload_done, s));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel sequence
+// ---------------------------------------------------------------------------
+template <int UNR>
+static void launch_stream_kernels_hist(const Pipeline& p, int64_t chunk, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_hist<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
+                     p.total, chunk, p.d_hist);
+}
+
+template <int UNR>
+static void launch_stream_kernels_compact(const Pipeline& p, int64_t chunk, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_compact<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
+                     p.total, chunk, p.d_ctl, p.d_cand);
+}
+
+int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm) {
+  const bool timed = tm && tm->on;
+  auto mark = [&](int i) -> int {
+    if (timed) RADIK_HIP(hipEventRecord(tm->ev[i], s));
+    return 0;
+  };
+  int rc;
+  if ((rc = mark(0))) return rc;
+  if (p.n_large) {
+    const int64_t want_blocks = (int64_t)f.cus * o.blocks_per_cu;
+    int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
+    const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
+    if (chunk < 4 * gran) chunk = 4 * gran;
+    chunk = (chunk + gran - 1) / gran * gran;
+    const int grid = (int)((p.total + chunk - 1) / chunk);
+    switch (o.unr) {
+      case 1: launch_stream_kernels_hist<1>(p, chunk, grid, s); break;
+      case 4: launch_stream_kernels_hist<4>(p, chunk, grid, s); break;
+      default: launch_stream_kernels_hist<2>(p, chunk, grid, s); break;
+    }
+    RADIK_HIP(hipGetLastError());
+    if ((rc = mark(1))) return rc;
+    hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl);
+    RADIK_HIP(hipGetLastError());
+    if ((rc = mark(2))) return rc;
+    switch (o.unr) {
+      case 1: launch_stream_kernels_compact<1>(p, chunk, grid, s); break;
+      case 4: launch_stream_kernels_compact<4>(p, chunk, grid, s); break;
+      default: launch_stream_kernels_compact<2>(p, chunk, grid, s); break;
+    }
+    RADIK_HIP(hipGetLastError());
+    if ((rc = mark(3))) return rc;
+    hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)f.refine_cap * 8, s,
+                       p.d_large, p.d_ctl, p.d_cand, f.refine_cap);
+    RADIK_HIP(hipGetLastError());
+    if ((rc = mark(4))) return rc;
+    hipLaunchKernelGGL(k_sort, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)f.sort_cap * 8, s, p.d_large,
+                       p.d_ctl, p.d_cand, f.sort_cap);
+    RADIK_HIP(hipGetLastError());
+  } else {
+    for (int i = 1; i <= 4; ++i)
+      if ((rc = mark(i))) return rc;
+  }
+  if ((rc = mark(5))) return rc;
+  if (p.n_small) {
+    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)f.small_cap * 16, s,
+                       p.d_small, f.small_cap);
+    RADIK_HIP(hipGetLastError());
+  }
+  if ((rc = mark(6))) return rc;
+  return 0;
+}
+
+// Diagnostics entry points' kernels.
+int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStream_t s) {
+  if (n <= 0) return 0;
+  const int grid = (int)((n + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_diag_scan, dim3(grid), dim3(BLOCK), 0, s, in, ow, ob, n);
+  RADIK_HIP(hipGetLastError());
+  return 0;
+}
+
+int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s) {
+  if (!p.n_large) return 0;
+  const int64_t want_blocks = (int64_t)f.cus * 8;
+  int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
+  const int64_t gran = (int64_t)BLOCK * 8;
+  if (chunk < 4 * gran) chunk = 4 * gran;
+  chunk = (chunk + gran - 1) / gran * gran;
+  const int grid = (int)((p.total + chunk - 1) / chunk);
+  launch_stream_kernels_hist<2>(p, chunk, grid, s);
+  RADIK_HIP(hipGetLastError());
+  RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl);
+  RADIK_HIP(hipGetLastError());
+  RADIK_HIP(hipMemcpy2DAsync(sel_out, 4 * sizeof(uint32_t), p.d_ctl, sizeof(TaskCtl), 4 * sizeof(uint32_t),
+                             (size_t)p.n_large, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+}  // namespace radik

@@ -1,0 +1,321 @@
+// radik_serve.cpp -- serving engine: continuous batching of top-k requests.
+//
+// submit() queues a request (a rows x cols row-major matrix on the device,
+// one task per row, its own k).  A dispatcher thread coalesces everything
+// pending into one launch whenever an in-flight slot is free and the batch
+// fits the slot's share of the device budget; each slot owns a stream and a
+// pipeline (descriptors, histograms, spill buffer), so up to max_inflight
+// batches overlap on the GPU.  A completer thread stamps each batch's
+// completion on the host clock when its event fires.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/radik.h"
+#include "radik_host.h"
+
+using namespace radik;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Request {
+  int64_t ticket;
+  const float* in;
+  int64_t rows, cols, k;
+  float* val_out;
+  int32_t* idx_out;
+  int64_t bytes;  // admission footprint
+  Clock::time_point t_submit;
+};
+
+struct Result {
+  bool done = false;
+  double latency_ms = 0;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  Pipeline pipe;
+  std::vector<TaskSpec> specs;
+  std::vector<int64_t> tickets;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct radik_server {
+  int device = 0;
+  int64_t budget = 0;
+  int64_t slot_budget = 0;
+  uint32_t flags = 0;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::mutex mu;
+  std::condition_variable cv_work;   // dispatcher: new request or free slot
+  std::condition_variable cv_done;   // waiters: a result completed
+  std::deque<Request> pending;
+  std::unordered_map<int64_t, Result> results;
+  int64_t next_ticket = 1;
+  bool stop = false;
+  int inflight = 0;
+  std::thread dispatcher, completer;
+  std::atomic<int64_t> launches{0}, rows_done{0}, max_coalesced{0};
+  int error = 0;
+  std::string error_msg;
+};
+
+static int64_t footprint(int64_t rows, int64_t cols) {
+  // spill buffer + histogram + descriptors per row (upper bound)
+  return rows * (cols * 8 + HIST_BINS * 4 + 2 * (int64_t)sizeof(TaskDesc) + (int64_t)sizeof(TaskCtl) + 8);
+}
+
+static void dispatch_loop(radik_server* s) {
+  (void)hipSetDevice(s->device);
+  DevFacts* f = nullptr;
+  if (dev_facts(&f)) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->error = RADIK_EHIP;
+    s->error_msg = "device query failed in dispatcher";
+    return;
+  }
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    s->cv_work.wait(lk, [&] {
+      return s->stop || (!s->pending.empty() && s->inflight < (int)s->slots.size());
+    });
+    if (s->stop && s->pending.empty()) return;
+    if (s->pending.empty()) continue;
+    Slot* slot = nullptr;
+    for (auto& sl : s->slots)
+      if (!sl->busy) { slot = sl.get(); break; }
+    if (!slot) continue;
+    // Coalesce everything pending that fits the slot budget (FIFO).
+    std::vector<Request> batch;
+    int64_t used = 0;
+    while (!s->pending.empty()) {
+      const Request& r = s->pending.front();
+      if (!batch.empty() && used + r.bytes > s->slot_budget) break;
+      used += r.bytes;
+      batch.push_back(r);
+      s->pending.pop_front();
+    }
+    slot->busy = true;
+    s->inflight++;
+    lk.unlock();
+    slot->specs.clear();
+    slot->tickets.clear();
+    const uint32_t tflags = s->flags & 0x0Fu & ~RADIK_WITHIDXIN;
+    int64_t nrows = 0;
+    for (const Request& r : batch) {
+      for (int64_t i = 0; i < r.rows; ++i) {
+        TaskSpec sp;
+        sp.in = r.in + i * r.cols;
+        sp.idx_in = nullptr;
+        sp.val_out = r.val_out + i * r.k;
+        sp.idx_out = r.idx_out + i * r.k;
+        sp.len = r.cols;
+        sp.k = r.k;
+        sp.flags = tflags;
+        slot->specs.push_back(sp);
+      }
+      nrows += r.rows;
+      slot->tickets.push_back(r.ticket);
+    }
+    LaunchOpts o;
+    int rc = stage_tasks(slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, slot->stream);
+    if (!rc) rc = upload(slot->pipe, slot->stream);
+    if (!rc) rc = enqueue(slot->pipe, *f, o, slot->stream, nullptr);
+    if (!rc && hipEventRecord(slot->done, slot->stream) != hipSuccess) rc = RADIK_EHIP;
+    s->launches++;
+    s->rows_done += nrows;
+    int64_t prev = s->max_coalesced.load();
+    while ((int64_t)batch.size() > prev && !s->max_coalesced.compare_exchange_weak(prev, (int64_t)batch.size())) {
+    }
+    lk.lock();
+    if (rc) {
+      s->error = rc;
+      s->error_msg = "batch launch failed";
+      // The completer still releases the slot once the stream drains.
+      (void)hipEventRecord(slot->done, slot->stream);
+    }
+    s->cv_work.notify_all();
+  }
+}
+
+static void complete_loop(radik_server* s) {
+  (void)hipSetDevice(s->device);
+  while (true) {
+    bool any = false;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (s->stop && s->inflight == 0 && s->pending.empty()) return;
+      for (auto& sl : s->slots) {
+        if (!sl->busy) continue;
+        any = true;
+        if (hipEventQuery(sl->done) != hipSuccess) continue;
+        const auto now = Clock::now();
+        for (int64_t t : sl->tickets) {
+          auto it = s->results.find(t);
+          if (it == s->results.end()) continue;
+          it->second.done = true;
+        }
+        sl->busy = false;
+        s->inflight--;
+        // latency: stamped against the submit time kept in the request map
+        for (int64_t t : sl->tickets) {
+          auto it = s->results.find(t);
+          if (it != s->results.end())
+            it->second.latency_ms = std::chrono::duration<double, std::milli>(now.time_since_epoch()).count() -
+                                    it->second.latency_ms;  // latency_ms held the submit stamp
+        }
+        s->cv_work.notify_all();
+        s->cv_done.notify_all();
+      }
+    }
+    if (!any) {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
+    } else {
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  }
+}
+
+extern "C" {
+
+int radik_server_create(radik_server** out, int64_t budget_bytes, int max_inflight, uint32_t flags) {
+  clear_error();
+  if (!out) return set_error(RADIK_EINVAL, "srv is NULL");
+  *out = nullptr;
+  if (budget_bytes <= 0 || max_inflight <= 0 || max_inflight > 16)
+    return set_error(RADIK_EINVAL, "budget must be > 0 and max_inflight in 1..16");
+  std::unique_ptr<radik_server> s(new (std::nothrow) radik_server());
+  if (!s) return set_error(RADIK_ENOMEM, "out of host memory");
+  RADIK_HIP(hipGetDevice(&s->device));
+  DevFacts* f = nullptr;
+  int rc = dev_facts(&f);
+  if (rc) return rc;
+  s->budget = budget_bytes;
+  s->slot_budget = budget_bytes / max_inflight;
+  s->flags = flags;
+  for (int i = 0; i < max_inflight; ++i) {
+    std::unique_ptr<Slot> sl(new Slot());
+    RADIK_HIP(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+    RADIK_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+    s->slots.push_back(std::move(sl));
+  }
+  radik_server* raw = s.get();
+  raw->dispatcher = std::thread(dispatch_loop, raw);
+  raw->completer = std::thread(complete_loop, raw);
+  *out = s.release();
+  return 0;
+}
+
+int radik_server_submit(radik_server* s, const float* in, int64_t rows, int64_t cols, int64_t k, float* val_out,
+                        int32_t* idx_out, int64_t* ticket) {
+  clear_error();
+  if (!s || !ticket) return set_error(RADIK_EINVAL, "NULL argument");
+  if (rows <= 0 || cols <= 0 || k < 0 || k > INT32_MAX || cols > INT32_MAX)
+    return set_error(RADIK_EINVAL, "bad request shape rows=%lld cols=%lld k=%lld", (long long)rows,
+                     (long long)cols, (long long)k);
+  if (!in || (k > 0 && (!val_out || !idx_out))) return set_error(RADIK_EINVAL, "NULL buffer");
+  const int64_t bytes = footprint(rows, cols);
+  if (bytes > s->slot_budget)
+    return set_error(RADIK_EADMIT, "request needs %lld bytes of workspace, a batch may use %lld", (long long)bytes,
+                     (long long)s->slot_budget);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return set_error(RADIK_EINVAL, "server is stopping");
+  Request r;
+  r.ticket = s->next_ticket++;
+  r.in = in;
+  r.rows = rows;
+  r.cols = cols;
+  r.k = k;
+  r.val_out = val_out;
+  r.idx_out = idx_out;
+  r.bytes = bytes;
+  r.t_submit = Clock::now();
+  Result res;
+  res.latency_ms = std::chrono::duration<double, std::milli>(r.t_submit.time_since_epoch()).count();
+  s->results[r.ticket] = res;
+  s->pending.push_back(r);
+  *ticket = r.ticket;
+  s->cv_work.notify_all();
+  return 0;
+}
+
+int radik_server_poll(radik_server* s, int64_t ticket, double* latency_ms) {
+  clear_error();
+  if (!s) return set_error(RADIK_EINVAL, "srv is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->error) return set_error(s->error, "%s", s->error_msg.c_str());
+  auto it = s->results.find(ticket);
+  if (it == s->results.end()) return set_error(RADIK_EINVAL, "unknown ticket %lld", (long long)ticket);
+  if (!it->second.done) return RADIK_ENOTREADY;
+  if (latency_ms) *latency_ms = it->second.latency_ms;
+  s->results.erase(it);
+  return 0;
+}
+
+int radik_server_wait(radik_server* s, int64_t ticket, double* latency_ms) {
+  clear_error();
+  if (!s) return set_error(RADIK_EINVAL, "srv is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->results.find(ticket);
+  if (it == s->results.end()) return set_error(RADIK_EINVAL, "unknown ticket %lld", (long long)ticket);
+  s->cv_done.wait(lk, [&] { return s->error != 0 || s->results[ticket].done; });
+  if (s->error) return set_error(s->error, "%s", s->error_msg.c_str());
+  it = s->results.find(ticket);
+  if (latency_ms) *latency_ms = it->second.latency_ms;
+  s->results.erase(it);
+  return 0;
+}
+
+int radik_server_drain(radik_server* s) {
+  clear_error();
+  if (!s) return set_error(RADIK_EINVAL, "srv is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->error != 0 || (s->pending.empty() && s->inflight == 0); });
+  if (s->error) return set_error(s->error, "%s", s->error_msg.c_str());
+  return 0;
+}
+
+int radik_server_stats(radik_server* s, int64_t* launches, int64_t* rows_done, int64_t* max_coalesced) {
+  clear_error();
+  if (!s) return set_error(RADIK_EINVAL, "srv is NULL");
+  if (launches) *launches = s->launches.load();
+  if (rows_done) *rows_done = s->rows_done.load();
+  if (max_coalesced) *max_coalesced = s->max_coalesced.load();
+  return 0;
+}
+
+int radik_server_destroy(radik_server* s) {
+  clear_error();
+  if (!s) return 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->dispatcher.joinable()) s->dispatcher.join();
+  if (s->completer.joinable()) s->completer.join();
+  for (auto& sl : s->slots) {
+    if (sl->stream) (void)hipStreamSynchronize(sl->stream);
+    if (sl->done) (void)hipEventDestroy(sl->done);
+    if (sl->stream) (void)hipStreamDestroy(sl->stream);
+  }
+  delete s;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+"""Persistent batched form and serving engine over libradik_hip.so.
+
+Plan: one pass sequence for many tasks of uneven length, each with its own k,
+replayed from a captured hipGraph (no reference counterpart: the reference
+launches one grid per pass, RadiK.jl src/algorithm.jl:221-334).
+Server: continuous batching of top-k requests against a device budget.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray
+
+
+class Plan:
+    def __init__(self, task_lens, k, largest=True, rev=False, withscale=False, indices_dtype=None,
+                 idx_dtype=np.int32, use_graph=True, stream=None):
+        lens = [int(x) for x in task_lens]
+        ks = [int(k)] * len(lens) if np.isscalar(k) else [int(x) for x in k]
+        if len(ks) != len(lens):
+            raise ValueError("k must be a scalar or one value per task")
+        self.task_lens = lens
+        self.ks = ks
+        self.idx_dtype = np.dtype(idx_dtype)
+        flags = _lib.RADIK_WITHPACKING
+        flags |= _lib.RADIK_LARGEST if largest else 0
+        flags |= 0 if rev else _lib.RADIK_ASCEND
+        flags |= _lib.RADIK_WITHSCALE if withscale else 0
+        flags |= _lib.RADIK_WITHIDXIN if indices_dtype is not None else 0
+        self.flags = flags
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().radik_plan_create(ctypes.byref(h), _lib.i64_array(lens), _lib.i64_array(ks),
+                                                 len(lens), flags, self.idx_dtype.itemsize,
+                                                 None if stream is None else stream.handle))
+        self.handle = h.value
+        _lib.check(_lib.load().radik_plan_set_graph(self.handle, 1 if use_graph else 0))
+
+    @property
+    def out_size(self):
+        return int(sum(self.ks))
+
+    def execute(self, val_out, idx_out, val_in, idx_in=None, stream=None):
+        _lib.check(_lib.load().radik_plan_execute(self.handle, val_out.ptr, idx_out.ptr, val_in.ptr,
+                                                  None if idx_in is None else idx_in.ptr,
+                                                  None if stream is None else stream.handle))
+
+    def set_timing(self, on):
+        _lib.check(_lib.load().radik_plan_set_timing(self.handle, 1 if on else 0))
+
+    def timing(self):
+        ms = (ctypes.c_double * 6)()
+        n = ctypes.c_int64()
+        _lib.check(_lib.load().radik_plan_timing(self.handle, ms, ctypes.byref(n)))
+        names = ("hist", "select", "compact", "refine", "sort", "small")
+        return {nm: ms[i] for i, nm in enumerate(names)}, n.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().radik_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Server:
+    def __init__(self, budget_bytes=16 << 30, max_inflight=4, largest=True, rev=True):
+        flags = (_lib.RADIK_LARGEST if largest else 0) | (0 if rev else _lib.RADIK_ASCEND) | _lib.RADIK_WITHPACKING
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().radik_server_create(ctypes.byref(h), int(budget_bytes), int(max_inflight), flags))
+        self.handle = h.value
+
+    def submit(self, x, rows, cols, k, val_out, idx_out):
+        t = ctypes.c_int64()
+        _lib.check(_lib.load().radik_server_submit(self.handle, x.ptr, int(rows), int(cols), int(k), val_out.ptr,
+                                                   idx_out.ptr, ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        lat = ctypes.c_double()
+        rc = _lib.load().radik_server_poll(self.handle, int(ticket), ctypes.byref(lat))
+        if rc == _lib.RADIK_ENOTREADY:
+            return None
+        _lib.check(rc)
+        return lat.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double()
+        _lib.check(_lib.load().radik_server_wait(self.handle, int(ticket), ctypes.byref(lat)))
+        return lat.value
+
+    def drain(self):
+        _lib.check(_lib.load().radik_server_drain(self.handle))
+
+    def stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.load().radik_server_stats(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"launches": a.value, "rows": b.value, "max_coalesced": c.value}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().radik_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,257 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle: bit-exact
+values, bit-exact indices (threshold ties to the lowest position).
+
+Cases: the committed golden fixtures, the reference's own end-to-end shapes
+(test/test_algorithm.jl) on seeded data, pass-level intermediates
+(test/test_count_bin.jl ramps, test/test_select_bin.jl), edge cases, and
+size-independent properties at full configuration sizes."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import radik
+from radik import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_topk(x, lens, k, flags, idx=None, idx_dtype=np.int32):
+    v, i = O.topk_exact(x, lens, k, flags, idx_in=idx, idx_dtype=idx_dtype)
+    return v.T, i.T  # [k, T] like the device outputs
+
+
+def run_ws(x, lens, k, flags, idx=None, idx_dtype=np.int32, vload=32, ws=None):
+    T = len(lens)
+    d_x = DeviceArray.from_host(np.asarray(x, np.float32))
+    d_i = DeviceArray.from_host(np.asarray(idx, idx_dtype)) if idx is not None else None
+    vo = DeviceArray.zeros((k, T), np.float32)
+    io = DeviceArray.zeros((k, T), idx_dtype)
+    if ws is None:
+        ws = radik.RadiKWorkspace(max(max(lens), 1), T)
+    radik.topk_radix_select_(vo, io, ws, d_x, d_i, lens, k, LARGEST=bool(flags & O.LARGEST),
+                             ASCEND=bool(flags & O.ASCEND), WITHSCALE=bool(flags & O.WITHSCALE),
+                             WITHIDXIN=idx is not None, VLOAD_SIZE_BYTES=vload)
+    return vo.to_host(), io.to_host()
+
+
+def assert_same(got, exp, name):
+    gv, gi = got
+    ev, ei = exp
+    assert gv.shape == ev.shape, name
+    bad = np.nonzero(gv.view(np.uint32) != ev.view(np.uint32))
+    assert bad[0].size == 0, f"{name}: {bad[0].size} value mismatches, first at {bad[0][0]},{bad[1][0]}"
+    bad = np.nonzero(gi.astype(np.int64) != ei.astype(np.int64))
+    assert bad[0].size == 0, f"{name}: {bad[0].size} index mismatches, first at {bad[0][0]},{bad[1][0]}"
+
+
+def test_device_is_gfx950():
+    info = radik.device_info()
+    say("device", info, radik.lds_caps())
+    assert info["arch"].startswith("gfx950")
+
+
+def test_wave_and_block_scan():
+    # counterpart of test/test_utils.jl:53-106 (@warpreduce) on wave64
+    import ctypes
+    from radik import _lib
+    for x in (np.arange(1, 257, dtype=np.uint32),
+              np.random.default_rng(0).integers(0, 1000, 4096).astype(np.uint32)):
+        n = x.size
+        d = DeviceArray.from_host(x)
+        w = DeviceArray.zeros(n, np.uint32)
+        b = DeviceArray.zeros(n, np.uint32)
+        _lib.check(_lib.load().radik_diag_scan(d.ptr, w.ptr, b.ptr, n, None))
+        ew = np.concatenate([np.cumsum(c) for c in x.reshape(-1, 64)]).astype(np.uint32)
+        eb = np.concatenate([np.cumsum(c) for c in x.reshape(-1, 256)]).astype(np.uint32)
+        assert np.array_equal(w.to_host(), ew)
+        assert np.array_equal(b.to_host(), eb)
+    assert w.to_host()[:64].tolist() == np.cumsum(np.arange(1, 65)).tolist()
+
+
+def _diag_pass1(x, lens, k, flags):
+    from radik import _lib
+    T = len(lens)
+    d = DeviceArray.from_host(np.asarray(x, np.float32))
+    h = DeviceArray.zeros(T * 4096, np.uint32)
+    s = DeviceArray.zeros(T * 4, np.uint32)
+    _lib.check(_lib.load().radik_diag_pass1(d.ptr, _lib.i64_array(lens), T, k, flags, h.ptr, s.ptr, None))
+    return h.to_host().reshape(T, 4096), s.to_host().reshape(T, 4)
+
+
+@pytest.mark.parametrize("flags", [O.LARGEST | O.WITHSCALE, O.LARGEST, 0, O.WITHSCALE])
+def test_pass1_histogram_ramps(known_answers, flags):
+    # test/test_count_bin.jl:11-61: the three ramps; exact integer histograms
+    ramps = known_answers["count_bin_ramps"]["ramps"]
+    parts = [np.arange(lo, hi + 1, dtype=np.float32) for lo, hi in ramps]
+    lens = [p.size for p in parts]
+    say("pass1 ramps", flags)
+    h, sel = _diag_pass1(np.concatenate(parts), lens, 100, flags)
+    for t, p in enumerate(parts):
+        exp = O.rank_histogram(p, flags, 20, 12)
+        assert np.array_equal(h[t].astype(np.int64), exp)
+        # bin selection on the same histogram (test/test_select_bin.jl rule)
+        b, krem, cnt = O.select_bin(exp, 100, False)
+        b -= 1
+        if krem == cnt:
+            assert sel[t, 0] == b + 1 and sel[t, 1] == 0xFFFFFFFF
+        else:
+            assert (sel[t, 0], sel[t, 1], sel[t, 2], sel[t, 3]) == (b, b, krem, cnt)
+
+
+def test_pass1_histogram_random_and_ties():
+    rng = np.random.default_rng(9)
+    lens = [1, 4097, 100000, 5, 300000, 0, 77777]
+    x = rng.standard_normal(sum(lens)).astype(np.float32)
+    x[:50000] = np.round(x[:50000] * 4) / 4
+    say("pass1 random")
+    h, _ = _diag_pass1(x, lens, 64, O.LARGEST)
+    off = 0
+    for t, n in enumerate(lens):
+        assert np.array_equal(h[t].astype(np.int64), O.rank_histogram(x[off:off + n], O.LARGEST, 20, 12))
+        off += n
+
+
+def test_golden_cases_workspace(golden_cases):
+    for c in golden_cases:
+        say("golden ws", c["name"])
+        got = run_ws(c["x"], list(c["lens"]), c["k"], c["flags"], idx=c["idx"])
+        assert_same(got, (c["vals"].T, c["inds"].T), c["name"])
+
+
+def test_golden_cases_plan(golden_cases):
+    for c in golden_cases:
+        if c["idx"] is not None:
+            continue
+        for use_graph in (True, False):
+            say("golden plan", c["name"], use_graph)
+            lens = list(c["lens"])
+            T, k = len(lens), c["k"]
+            fl = c["flags"]
+            p = radik.Plan(lens, k, largest=bool(fl & O.LARGEST), rev=not (fl & O.ASCEND),
+                           withscale=bool(fl & O.WITHSCALE), use_graph=use_graph)
+            d_x = DeviceArray.from_host(c["x"])
+            vo = DeviceArray.zeros((k, T), np.float32)
+            io = DeviceArray.zeros((k, T), np.int32)
+            for rep in range(3):  # graph replays must all match
+                vo.fill(np.float32(-7))
+                p.execute(vo, io, d_x)
+                radik.synchronize()
+                assert_same((vo.to_host(), io.to_host()), (c["vals"].T, c["inds"].T), f"{c['name']} rep{rep}")
+
+
+@pytest.mark.parametrize("largest", [True, False])
+@pytest.mark.parametrize("rev", [True, False])
+def test_reference_basic_grid(largest, rev):
+    # test/test_algorithm.jl:6-40: n = 2^14, K in 16..4096, largest x rev
+    rng = np.random.default_rng(42 + 2 * largest + rev)
+    n = 1 << 14
+    for k in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096):
+        x = rng.random(n, dtype=np.float32)
+        say("basic", largest, rev, k)
+        d = DeviceArray.from_host(x)
+        v, i = radik.topk(d, k, largest=largest, rev=rev)
+        v, i = v.to_host(), i.to_host()
+        flags = (O.LARGEST if largest else 0) | (0 if rev else O.ASCEND)
+        ev, ei = oracle_topk(x, [n], k, flags)
+        assert np.array_equal(v, ev[:, 0]) and np.array_equal(i, ei[:, 0])
+        # the reference's own assertions
+        exp = np.sort(np.partition(x, n - k)[n - k:] if largest else np.partition(x, k - 1)[:k])
+        assert np.array_equal(v, exp[::-1] if rev else exp)
+        assert np.all((1 <= i) & (i <= n)) and len(np.unique(i)) == k
+        assert np.array_equal(x[i - 1], v)
+
+
+def test_reference_multitask_workspace_reuse():
+    # test/test_algorithm.jl:42-96: reused workspace, global idx_in
+    lens = [1000, 2000, 1500, 500, 5000]
+    k = 64
+    ws = radik.RadiKWorkspace(5000, 5, np.int32)
+    rng = np.random.default_rng(4)
+    for rep in range(2):
+        x = rng.random(sum(lens), dtype=np.float32)
+        idx = np.arange(1, sum(lens) + 1, dtype=np.int32)
+        say("multitask", rep)
+        got = run_ws(x, lens, k, O.LARGEST | O.ASCEND, idx=idx, ws=ws)
+        assert_same(got, oracle_topk(x, lens, k, O.LARGEST | O.ASCEND, idx=idx), "multitask")
+        off = 0
+        for t, n in enumerate(lens):
+            part = x[off:off + n]
+            assert np.array_equal(got[0][:, t], np.sort(np.partition(part, n - k)[n - k:]))
+            assert np.array_equal(x[got[1][:, t] - 1], got[0][:, t])
+            off += n
+
+
+def test_reference_overloads_and_batch():
+    # test/test_algorithm.jl:98-150
+    rng = np.random.default_rng(5)
+    x = rng.random(1000, dtype=np.float32)
+    d = DeviceArray.from_host(x)
+    v1, i1 = radik.topk(d, 100, largest=True, rev=False)
+    ind = DeviceArray.from_host(np.arange(1, 1001, dtype=np.int32))
+    v2, i2 = radik.topk(d, 100, indices=ind, largest=True, rev=False)
+    assert np.array_equal(v1.to_host(), v2.to_host()) and np.array_equal(i1.to_host(), i2.to_host())
+    lens = [1000, 1500, 800]
+    xb = rng.random(sum(lens), dtype=np.float32)
+    db = DeviceArray.from_host(xb)
+    va, ia = radik.topk(db, lens, 50)
+    seq = DeviceArray.from_host(np.concatenate([np.arange(1, n + 1, dtype=np.int32) for n in lens]))
+    vb, ib = radik.topk(db, lens, 50, indices=seq)
+    assert np.array_equal(va.to_host(), vb.to_host()) and np.array_equal(ia.to_host(), ib.to_host())
+    assert_same((va.to_host(), ia.to_host()), oracle_topk(xb, lens, 50, O.LARGEST | O.ASCEND), "batch")
+
+
+def test_reference_errors_on_gpu():
+    # test/test_algorithm.jl:259-269
+    d = DeviceArray.from_host(np.random.default_rng(0).standard_normal(1000).astype(np.float32))
+    with pytest.raises(radik.RadiKError):
+        radik.topk_(DeviceArray.zeros((50, 1), np.float32), DeviceArray.zeros((50, 1), np.int32), d, [1000], 100)
+    with pytest.raises(radik.RadiKError):
+        radik.topk(d, 100, indices=DeviceArray.from_host(np.arange(1, 501, dtype=np.int32)))
+
+
+def test_edge_cases():
+    rng = np.random.default_rng(6)
+    cases = [
+        ("empty_and_tiny", rng.standard_normal(3).astype(np.float32), [0, 1, 2, 0], 2),
+        ("k_equals_n", rng.standard_normal(5000).astype(np.float32), [5000], 5000),
+        ("k_one_ties", np.full(9000, 1.5, np.float32), [9000], 1),
+        ("all_equal_large", np.full(300000, -2.0, np.float32), [300000], 1000),
+        ("signed_zeros", np.tile(np.array([0.0, -0.0], np.float32), 6000), [12000], 6001),
+        ("int64_idx", rng.standard_normal(20000).astype(np.float32), [20000], 100),
+    ]
+    for name, x, lens, k in cases:
+        for flags in (O.LARGEST | O.ASCEND, 0):
+            say("edge", name, flags)
+            idt = np.int64 if name == "int64_idx" else np.int32
+            got = run_ws(x, lens, k, flags, idx_dtype=idt)
+            assert_same(got, oracle_topk(x, lens, k, flags, idx_dtype=idt), name)
+
+
+def test_vload_widths_and_determinism():
+    rng = np.random.default_rng(8)
+    lens = [70001, 123457, 9999]
+    x = (np.round(rng.standard_normal(sum(lens)) * 8) / 8).astype(np.float32)
+    exp = oracle_topk(x, lens, 300, O.LARGEST | O.ASCEND)
+    for vload in (4, 16, 32, 64):
+        say("vload", vload)
+        assert_same(run_ws(x, lens, 300, O.LARGEST | O.ASCEND, vload=vload), exp, f"vload{vload}")
+
+
+def test_large_single_task_properties():
+    # n = 2^24 (scaled from the 2^28 configuration): exact against the oracle
+    n = 1 << 24
+    rng = np.random.default_rng(2828)
+    x = rng.standard_normal(n, dtype=np.float32)
+    perm = rng.permutation(n).astype(np.int32)
+    for k in (16, 1024, 4096):
+        for flags in (O.LARGEST, O.LARGEST | O.ASCEND, 0):
+            say("large", k, flags)
+            got = run_ws(x, [n], k, flags, idx=perm)
+            assert_same(got, oracle_topk(x, [n], k, flags, idx=perm), f"2^24 k{k} f{flags}")
