@@ -100,7 +100,7 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
   if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
   // The pinned arrays may still feed an in-flight copy.
   RADIK_HIP(hipEventSynchronize(upload_done));
-  if ((size_t)nl > cap_large) {
+  if (!h_large || (size_t)nl > cap_large) {  // the prefix arrays always need their sentinel
     // Device buffers may be in use by queued work on s.
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = grow(cap_large, (size_t)nl);
@@ -159,10 +159,16 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
   return 0;
 }
 
+// Short tasks run whole in one block (k_small); small_cap 0 routes every
+// task through the streaming passes.
+static inline bool is_short(int64_t len, const DevFacts& f) {
+  return f.small_cap > 0 && len <= (int64_t)f.small_cap;
+}
+
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s) {
   int64_t nl = 0, ns = 0, tot = 0;
   for (int64_t t = 0; t < n; ++t) {
-    if (tasks[t].len <= (int64_t)f.small_cap) ++ns;
+    if (is_short(tasks[t].len, f)) ++ns;
     else { ++nl; tot += tasks[t].len; }
   }
   int rc = p.reserve(nl, ns, tot, s);
@@ -178,7 +184,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     d.len = ts.len;
     d.k = (int32_t)ts.k;
     d.flags = ts.flags;
-    if (ts.len <= (int64_t)f.small_cap) {
+    if (is_short(ts.len, f)) {
       d.base = 0;
       p.h_small[is++] = d;
     } else {
@@ -235,7 +241,7 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
     const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
     if (chunk < 4 * gran) chunk = 4 * gran;
     chunk = (chunk + gran - 1) / gran * gran;
-    const int grid = (int)((p.total + chunk - 1) / chunk);
+    const int grid = (int)std::max<int64_t>(1, (p.total + chunk - 1) / chunk);
     switch (o.unr) {
       case 1: launch_stream_kernels_hist<1>(p, chunk, grid, s); break;
       case 4: launch_stream_kernels_hist<4>(p, chunk, grid, s); break;
@@ -290,7 +296,7 @@ int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel
   const int64_t gran = (int64_t)BLOCK * 8;
   if (chunk < 4 * gran) chunk = 4 * gran;
   chunk = (chunk + gran - 1) / gran * gran;
-  const int grid = (int)((p.total + chunk - 1) / chunk);
+  const int grid = (int)std::max<int64_t>(1, (p.total + chunk - 1) / chunk);
   launch_stream_kernels_hist<2>(p, chunk, grid, s);
   RADIK_HIP(hipGetLastError());
   RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),

@@ -72,7 +72,8 @@ def test_wave_and_block_scan():
         eb = np.concatenate([np.cumsum(c) for c in x.reshape(-1, 256)]).astype(np.uint32)
         assert np.array_equal(w.to_host(), ew)
         assert np.array_equal(b.to_host(), eb)
-    assert w.to_host()[:64].tolist() == np.cumsum(np.arange(1, 65)).tolist()
+        if x[0] == 1:
+            assert w.to_host()[:64].tolist() == np.cumsum(np.arange(1, 65)).tolist()
 
 
 def _diag_pass1(x, lens, k, flags):
