@@ -97,10 +97,10 @@ def time_plan(radik, plan, vo, io, d_x, steps, warmup):
     return e0.elapsed_ms(e1) / steps, wall / steps * 1e3
 
 
-def phase_times(radik, plan, vo, io, d_x, steps):
+def phase_times(radik, plan, vo, io, d_x, steps, d_i=None):
     plan.set_timing(True)
     for _ in range(steps):
-        plan.execute(vo, io, d_x)
+        plan.execute(vo, io, d_x, d_i)
     radik.synchronize()
     ms, n = plan.timing()
     plan.set_timing(False)
@@ -168,7 +168,7 @@ def secondary(radik, O, warmup, steps):
         e1.record()
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / steps
-        ph, cnt = phase_times(radik, p, vo, io, d, steps) if k == 1024 else (None, 0)
+        ph, cnt = phase_times(radik, p, vo, io, d, steps, di)
         B = 8.0 * n + 8.0 * k
         r = {"ms": round(ms, 4), "Gelem_s": round(n / (ms * 1e-3) / 1e9, 1),
              "path_hbm_frac": round(B / (ms * 1e-3) / HBM_PEAK, 4)}
@@ -192,7 +192,13 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="time the batched plan only (profiling runs)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run the plain copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.only_batched:
+        args.no_secondary = args.no_cpu = args.no_plain = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the radix top-k path runs on a single MI355X here "
               "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
@@ -204,6 +210,15 @@ def main():
 
     info = radik.device_info()
     log("device", info, radik.lds_caps())
+    if args.copy_bytes:
+        from radik import _lib
+        nb = args.copy_bytes // 16 * 16
+        src = DeviceArray.zeros(nb // 4, np.float32)
+        dst = DeviceArray.zeros(nb // 4, np.float32)
+        for _ in range(5):
+            _lib.check(_lib.load().radik_diag_copy(dst.ptr, src.ptr, nb, None))
+        del src, dst
+        log(f"copy kernel: 5 x {nb} bytes")
     t0 = time.perf_counter()
     x, lens = uneven_batch()
     N = int(lens.sum())

@@ -16,6 +16,7 @@ namespace radik {
 const char* last_error();
 int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStream_t s);
 int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s);
+int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 }  // namespace radik
 
 using namespace radik;
@@ -52,7 +53,7 @@ struct radik_plan {
   hipStream_t graph_stream = nullptr;
   // timing
   PhaseTimer timer;
-  double ms[6] = {0, 0, 0, 0, 0, 0};
+  double ms[NUM_PHASES] = {};
   int64_t timed = 0;
   ~radik_plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -286,11 +287,11 @@ int radik_plan_set_timing(radik_plan* p, int enable) {
   return 0;
 }
 
-int radik_plan_timing(radik_plan* p, double* ms6, int64_t* count) {
+int radik_plan_timing(radik_plan* p, double* ms8, int64_t* count) {
   clear_error();
   if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
-  if (ms6)
-    for (int i = 0; i < 6; ++i) ms6[i] = p->ms[i];
+  if (ms8)
+    for (int i = 0; i < NUM_PHASES; ++i) ms8[i] = p->ms[i];
   if (count) *count = p->timed;
   return 0;
 }
@@ -335,8 +336,8 @@ int radik_plan_execute(radik_plan* p, float* val_out, void* idx_out, const float
   if (p->timer.on) {
     rc = enqueue(p->pipe, *f, p->opts, s, &p->timer);
     if (rc) return rc;
-    RADIK_HIP(hipEventSynchronize(p->timer.ev[6]));
-    for (int i = 0; i < 6; ++i) {
+    RADIK_HIP(hipEventSynchronize(p->timer.ev[NUM_PHASES]));
+    for (int i = 0; i < NUM_PHASES; ++i) {
       float ms = 0;
       RADIK_HIP(hipEventElapsedTime(&ms, p->timer.ev[i], p->timer.ev[i + 1]));
       p->ms[i] += ms;
@@ -432,6 +433,16 @@ int radik_diag_pass1(const float* val_in, const int64_t* task_lens, int64_t num_
   rc = diag_pass1(pipe, all, hist_out, sel_out, s);
   if (rc) return rc;
   RADIK_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int radik_diag_copy(void* dst, const void* src, int64_t bytes, void* stream) {
+  clear_error();
+  if (bytes < 0 || (bytes > 0 && (!dst || !src)) || (bytes % 16) != 0)
+    return set_error(RADIK_EINVAL, "bytes must be a multiple of 16 with non-NULL buffers");
+  int rc = diag_copy(dst, src, bytes, (hipStream_t)stream);
+  if (rc) return rc;
+  RADIK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return 0;
 }
 

@@ -45,18 +45,22 @@ struct LaunchOpts {
 struct Pipeline {
   // host copies (pinned)
   TaskDesc* h_large = nullptr;
-  int64_t* h_bases = nullptr;
+  int64_t* h_bases = nullptr;   // element prefix sums of the long tasks
+  int64_t* h_sbases = nullptr;  // sample-group prefix sums of the long tasks
   TaskDesc* h_small = nullptr;
   // device
   TaskDesc* d_large = nullptr;
   int64_t* d_bases = nullptr;
+  int64_t* d_sbases = nullptr;
   TaskDesc* d_small = nullptr;
   TaskCtl* d_ctl = nullptr;
-  uint32_t* d_hist = nullptr;
-  uint64_t* d_cand = nullptr;
+  uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
+  uint32_t* d_shist = nullptr;  // sample histograms, zero between launches
+  uint64_t* d_cand = nullptr;   // spill regions
   size_t cap_large = 0, cap_small = 0, cap_hist = 0, cap_cand = 0;
   // current content
-  int64_t n_large = 0, n_small = 0, total = 0;
+  int64_t n_large = 0, n_small = 0, total = 0, total_groups = 0;
+  uint32_t max_m = 0;  // largest min(k, len) over the long tasks
   hipEvent_t upload_done = nullptr;  // pinned buffers reusable after this
   ~Pipeline();
   int reserve(int64_t n_large, int64_t n_small, int64_t total_large, hipStream_t s);
@@ -76,8 +80,10 @@ struct TaskSpec {
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s);
 
 // Phase timing hooks (events recorded between kernels when non-null).
+// Phases: sample, predict, hist, select, compact, refine, sort, small.
+constexpr int NUM_PHASES = 8;
 struct PhaseTimer {
-  hipEvent_t ev[7] = {};
+  hipEvent_t ev[NUM_PHASES + 1] = {};
   bool on = false;
 };
 

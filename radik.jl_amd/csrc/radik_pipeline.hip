@@ -54,7 +54,9 @@ int dev_facts(DevFacts** out) {
       const int want = (int)budget;
       bool ok = hipFuncSetAttribute((const void*)k_refine, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     want - 20 * 1024) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                hipFuncSetAttribute((const void*)k_sort<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_sort<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     want) == hipSuccess &&
                 hipFuncSetAttribute((const void*)k_small, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     want - 20 * 1024) == hipSuccess;
@@ -66,7 +68,10 @@ int dev_facts(DevFacts** out) {
     f.lds_per_block = budget;
     const size_t stat = 20 * 1024;  // static LDS of k_refine / k_small (histogram + scratch)
     f.refine_cap = (uint32_t)((budget - stat) / 8);
-    f.sort_cap = (uint32_t)(budget / 8);
+    // register-blocked sort: up to 64 per thread, LDS scratch of a power of two
+    uint32_t sc = BLOCK;
+    while (sc * 2 <= (uint32_t)(budget / 8) && sc * 2 <= (uint32_t)BLOCK * 64) sc *= 2;
+    f.sort_cap = sc;
     f.small_cap = (uint32_t)((budget - stat) / 16);
     f.device = dev;
   }
@@ -80,6 +85,9 @@ int dev_facts(DevFacts** out) {
 Pipeline::~Pipeline() {
   if (h_large) (void)hipHostFree(h_large);
   if (h_bases) (void)hipHostFree(h_bases);
+  if (h_sbases) (void)hipHostFree(h_sbases);
+  if (d_sbases) (void)hipFree(d_sbases);
+  if (d_shist) (void)hipFree(d_shist);
   if (h_small) (void)hipHostFree(h_small);
   if (d_large) (void)hipFree(d_large);
   if (d_bases) (void)hipFree(d_bases);
@@ -100,20 +108,25 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
   if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
   // The pinned arrays may still feed an in-flight copy.
   RADIK_HIP(hipEventSynchronize(upload_done));
-  if ((size_t)nl > cap_large) {
+  if (!h_large || (size_t)nl > cap_large) {  // the prefix arrays always need their sentinel
     // Device buffers may be in use by queued work on s.
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = grow(cap_large, (size_t)nl);
     if (h_large) RADIK_HIP(hipHostFree(h_large));
     if (h_bases) RADIK_HIP(hipHostFree(h_bases));
+    if (h_sbases) RADIK_HIP(hipHostFree(h_sbases));
     if (d_large) RADIK_HIP(hipFree(d_large));
     if (d_bases) RADIK_HIP(hipFree(d_bases));
+    if (d_sbases) RADIK_HIP(hipFree(d_sbases));
     if (d_ctl) RADIK_HIP(hipFree(d_ctl));
-    h_large = nullptr; h_bases = nullptr; d_large = nullptr; d_bases = nullptr; d_ctl = nullptr;
+    h_large = nullptr; h_bases = nullptr; h_sbases = nullptr;
+    d_large = nullptr; d_bases = nullptr; d_sbases = nullptr; d_ctl = nullptr;
     if (hipHostMalloc((void**)&h_large, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_bases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_sbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&d_large, c * sizeof(TaskDesc)) != hipSuccess ||
         hipMalloc((void**)&d_bases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc((void**)&d_sbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_ctl, c * sizeof(TaskCtl)) != hipSuccess) {
       cap_large = 0;
       return set_error(6, "allocation of %zu task descriptors failed", c);
@@ -124,12 +137,16 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = grow(cap_hist, (size_t)nl);
     if (d_hist) RADIK_HIP(hipFree(d_hist));
+    if (d_shist) RADIK_HIP(hipFree(d_shist));
     d_hist = nullptr;
-    if (hipMalloc((void**)&d_hist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess) {
+    d_shist = nullptr;
+    if (hipMalloc((void**)&d_hist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&d_shist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess) {
       cap_hist = 0;
       return set_error(6, "allocation of %zu task histograms failed", c);
     }
     RADIK_HIP(hipMemset(d_hist, 0, c * HIST_BINS * sizeof(uint32_t)));
+    RADIK_HIP(hipMemset(d_shist, 0, c * HIST_BINS * sizeof(uint32_t)));
     cap_hist = c;
   }
   if ((size_t)ns > cap_small) {
@@ -159,15 +176,34 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
   return 0;
 }
 
+// Sample groups (of SAMPLE_GROUP elements) read to predict a long task's
+// first-digit bound: about 1/64 of the task, 16..1024 groups.  Tasks shorter
+// than 2^16 elements, or whose k exceeds 1/64 of their length, are not
+// predicted and take the two-read schedule.
+static int64_t sample_groups(int64_t len, int64_t k) {
+  const int64_t m = k < len ? k : len;
+  if (len < (int64_t)1 << 16 || m <= 0 || m * 64 > len) return 0;
+  int64_t g = len / ((int64_t)SAMPLE_GROUP * 64);
+  if (g < 16) g = 16;
+  if (g > 1024) g = 1024;
+  return g;
+}
+
+// Short tasks run whole in one block (k_small); small_cap 0 routes every
+// task through the streaming passes.
+static inline bool is_short(int64_t len, const DevFacts& f) {
+  return f.small_cap > 0 && len <= (int64_t)f.small_cap;
+}
+
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s) {
   int64_t nl = 0, ns = 0, tot = 0;
   for (int64_t t = 0; t < n; ++t) {
-    if (tasks[t].len <= (int64_t)f.small_cap) ++ns;
+    if (is_short(tasks[t].len, f)) ++ns;
     else { ++nl; tot += tasks[t].len; }
   }
   int rc = p.reserve(nl, ns, tot, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, base = 0;
+  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0;
   for (int64_t t = 0; t < n; ++t) {
     const TaskSpec& ts = tasks[t];
     TaskDesc d;
@@ -178,20 +214,27 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     d.len = ts.len;
     d.k = (int32_t)ts.k;
     d.flags = ts.flags;
-    if (ts.len <= (int64_t)f.small_cap) {
+    if (is_short(ts.len, f)) {
       d.base = 0;
       p.h_small[is++] = d;
     } else {
       d.base = base;
       p.h_bases[il] = base;
+      p.h_sbases[il] = groups;
+      const int64_t m = ts.k < ts.len ? ts.k : ts.len;
+      if (m > max_m) max_m = m;
       p.h_large[il++] = d;
       base += ts.len;
+      groups += sample_groups(ts.len, ts.k);
     }
   }
   p.h_bases[il] = base;
+  p.h_sbases[il] = groups;
   p.n_large = nl;
   p.n_small = ns;
   p.total = tot;
+  p.total_groups = groups;
+  p.max_m = (uint32_t)max_m;
   return 0;
 }
 
@@ -199,6 +242,7 @@ int upload(Pipeline& p, hipStream_t s) {
   if (p.n_large) {
     RADIK_HIP(hipMemcpyAsync(p.d_large, p.h_large, p.n_large * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_bases, p.h_bases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   }
   if (p.n_small)
     RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
@@ -212,7 +256,7 @@ int upload(Pipeline& p, hipStream_t s) {
 template <int UNR>
 static void launch_stream_kernels_hist(const Pipeline& p, int64_t chunk, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_hist<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     p.total, chunk, p.d_hist);
+                     p.total, chunk, p.d_hist, p.d_ctl, p.d_cand);
 }
 
 template <int UNR>
@@ -221,56 +265,89 @@ static void launch_stream_kernels_compact(const Pipeline& p, int64_t chunk, int 
                      p.total, chunk, p.d_ctl, p.d_cand);
 }
 
+static int64_t stream_chunk(const Pipeline& p, const DevFacts& f, const LaunchOpts& o, int* grid) {
+  const int64_t want_blocks = (int64_t)f.cus * o.blocks_per_cu;
+  int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
+  const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
+  if (chunk < 4 * gran) chunk = 4 * gran;
+  chunk = (chunk + gran - 1) / gran * gran;
+  *grid = (int)std::max<int64_t>(1, (p.total + chunk - 1) / chunk);
+  return chunk;
+}
+
 int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm) {
   const bool timed = tm && tm->on;
-  auto mark = [&](int i) -> int {
-    if (timed) RADIK_HIP(hipEventRecord(tm->ev[i], s));
+  int phase = 0;
+  auto mark = [&]() -> int {  // closes the current phase
+    if (timed) RADIK_HIP(hipEventRecord(tm->ev[phase], s));
+    ++phase;
     return 0;
   };
   int rc;
-  if ((rc = mark(0))) return rc;
+  if ((rc = mark())) return rc;  // start
   if (p.n_large) {
-    const int64_t want_blocks = (int64_t)f.cus * o.blocks_per_cu;
-    int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
-    const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
-    if (chunk < 4 * gran) chunk = 4 * gran;
-    chunk = (chunk + gran - 1) / gran * gran;
-    const int grid = (int)((p.total + chunk - 1) / chunk);
+    int grid = 0;
+    const int64_t chunk = stream_chunk(p, f, o, &grid);
+    if (p.total_groups > 0) {
+      const int sgrid = (int)((p.total_groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK);
+      hipLaunchKernelGGL(k_sample, dim3(sgrid), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
+                         p.total_groups, p.d_shist);
+      RADIK_HIP(hipGetLastError());
+    }
+    if ((rc = mark())) return rc;
+    hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
+                       p.d_ctl);
+    RADIK_HIP(hipGetLastError());
+    if ((rc = mark())) return rc;
     switch (o.unr) {
       case 1: launch_stream_kernels_hist<1>(p, chunk, grid, s); break;
       case 4: launch_stream_kernels_hist<4>(p, chunk, grid, s); break;
       default: launch_stream_kernels_hist<2>(p, chunk, grid, s); break;
     }
     RADIK_HIP(hipGetLastError());
-    if ((rc = mark(1))) return rc;
+    if ((rc = mark())) return rc;
     hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl);
     RADIK_HIP(hipGetLastError());
-    if ((rc = mark(2))) return rc;
+    if ((rc = mark())) return rc;
     switch (o.unr) {
       case 1: launch_stream_kernels_compact<1>(p, chunk, grid, s); break;
       case 4: launch_stream_kernels_compact<4>(p, chunk, grid, s); break;
       default: launch_stream_kernels_compact<2>(p, chunk, grid, s); break;
     }
     RADIK_HIP(hipGetLastError());
-    if ((rc = mark(3))) return rc;
+    if ((rc = mark())) return rc;
     hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)f.refine_cap * 8, s,
                        p.d_large, p.d_ctl, p.d_cand, f.refine_cap);
     RADIK_HIP(hipGetLastError());
-    if ((rc = mark(4))) return rc;
-    hipLaunchKernelGGL(k_sort, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)f.sort_cap * 8, s, p.d_large,
-                       p.d_ctl, p.d_cand, f.sort_cap);
+    if ((rc = mark())) return rc;
+    {
+      // results per thread of the register sort: the launch's largest m
+      uint32_t e = 1;
+      while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;
+      const size_t lds = (size_t)BLOCK * e * 8;
+      const dim3 g((unsigned)p.n_large), b(BLOCK);
+      switch (e) {
+        case 1: hipLaunchKernelGGL(k_sort<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 2: hipLaunchKernelGGL(k_sort<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 4: hipLaunchKernelGGL(k_sort<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 8: hipLaunchKernelGGL(k_sort<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 16: hipLaunchKernelGGL(k_sort<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 32: hipLaunchKernelGGL(k_sort<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        default: hipLaunchKernelGGL(k_sort<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+      }
+    }
     RADIK_HIP(hipGetLastError());
   } else {
-    for (int i = 1; i <= 4; ++i)
-      if ((rc = mark(i))) return rc;
+    for (int i = 0; i < 6; ++i)
+      if ((rc = mark())) return rc;
   }
-  if ((rc = mark(5))) return rc;
+  if ((rc = mark())) return rc;  // end of sort
   if (p.n_small) {
     hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)f.small_cap * 16, s,
                        p.d_small, f.small_cap);
     RADIK_HIP(hipGetLastError());
   }
-  if ((rc = mark(6))) return rc;
+  if ((rc = mark())) return rc;  // end of small
   return 0;
 }
 
@@ -283,14 +360,33 @@ int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStre
   return 0;
 }
 
+int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
+  const int64_t n16 = bytes / 16;
+  if (n16 <= 0) return 0;
+  int dev = 0, cus = 0;
+  RADIK_HIP(hipGetDevice(&dev));
+  RADIK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t want = (int64_t)(cus > 0 ? cus : 256) * 16;
+  const int grid = (int)std::min<int64_t>(want, (n16 + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_diag_copy, dim3(grid), dim3(BLOCK), 0, s, (const f32x4*)src, (f32x4*)dst, n16);
+  RADIK_HIP(hipGetLastError());
+  return 0;
+}
+
 int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s) {
   if (!p.n_large) return 0;
-  const int64_t want_blocks = (int64_t)f.cus * 8;
-  int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
-  const int64_t gran = (int64_t)BLOCK * 8;
-  if (chunk < 4 * gran) chunk = 4 * gran;
-  chunk = (chunk + gran - 1) / gran * gran;
-  const int grid = (int)((p.total + chunk - 1) / chunk);
+  LaunchOpts o;
+  int grid = 0;
+  const int64_t chunk = stream_chunk(p, f, o, &grid);
+  // Control blocks without a prediction: pass A only counts.
+  RADIK_HIP(hipStreamSynchronize(s));  // the pinned arrays may still feed the upload
+  p.total_groups = 0;
+  for (int64_t t = 0; t <= p.n_large; ++t) p.h_sbases[t] = 0;
+  RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  RADIK_HIP(hipStreamSynchronize(s));
+  hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
+                     p.d_ctl);
+  RADIK_HIP(hipGetLastError());
   launch_stream_kernels_hist<2>(p, chunk, grid, s);
   RADIK_HIP(hipGetLastError());
   RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),

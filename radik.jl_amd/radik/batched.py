@@ -49,10 +49,10 @@ class Plan:
         _lib.check(_lib.load().radik_plan_set_timing(self.handle, 1 if on else 0))
 
     def timing(self):
-        ms = (ctypes.c_double * 6)()
+        ms = (ctypes.c_double * 8)()
         n = ctypes.c_int64()
         _lib.check(_lib.load().radik_plan_timing(self.handle, ms, ctypes.byref(n)))
-        names = ("hist", "select", "compact", "refine", "sort", "small")
+        names = ("sample", "predict", "hist", "select", "compact", "refine", "sort", "small")
         return {nm: ms[i] for i, nm in enumerate(names)}, n.value
 
     def __del__(self):
