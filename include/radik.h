@@ -87,6 +87,13 @@ int radik_plan_execute(radik_plan *plan, float *val_out, void *idx_out, const fl
                        const void *idx_in, void *stream);
 /* use_graph = 0 launches the kernels eagerly (same results). */
 int radik_plan_set_graph(radik_plan *plan, int use_graph);
+/* Tuning knobs (results do not depend on them): vload_bytes 16/32/64 (bytes
+ * in flight per lane per step of the streaming passes, 0 = keep), streaming
+ * blocks per CU (0 = keep), longest task run whole in one block, candidates
+ * held in LDS by the refinement (0 = device defaults), and whether long tasks
+ * are sampled so one read of the input suffices. */
+int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu, int small_cap, int refine_cap,
+                           int predict);
 /* Per-kernel timing: when enabled, execute runs eagerly and records HIP events
  * around each kernel on the plan's stream; radik_plan_timing returns the
  * accumulated milliseconds of the 8 phases (sample, predict, hist, select,

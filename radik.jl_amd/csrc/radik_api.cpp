@@ -144,7 +144,7 @@ static int run_tasks(Pipeline& pipe, std::vector<TaskSpec>& specs, const LaunchO
   DevFacts* f = nullptr;
   int rc = dev_facts(&f);
   if (rc) return rc;
-  rc = stage_tasks(pipe, specs.data(), (int64_t)specs.size(), *f, s);
+  rc = stage_tasks(pipe, specs.data(), (int64_t)specs.size(), *f, o, s);
   if (rc) return rc;
   rc = upload(pipe, s);
   if (rc) return rc;
@@ -276,6 +276,21 @@ int radik_plan_set_graph(radik_plan* p, int use_graph) {
   return 0;
 }
 
+int radik_plan_set_options(radik_plan* p, int vload_bytes, int blocks_per_cu, int small_cap, int refine_cap,
+                           int predict) {
+  clear_error();
+  if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
+  if (blocks_per_cu < 0 || blocks_per_cu > 64 || small_cap < 0 || refine_cap < 0)
+    return set_error(RADIK_EINVAL, "bad plan options");
+  if (vload_bytes > 0) p->opts = opts_for(p->flags | RADIK_WITHPACKING, vload_bytes);
+  if (blocks_per_cu > 0) p->opts.blocks_per_cu = blocks_per_cu;
+  p->opts.small_cap = (uint32_t)small_cap;
+  p->opts.refine_cap = (uint32_t)refine_cap;
+  p->opts.predict = predict != 0;
+  p->staged = false;  // re-stage (task order may change) and re-capture
+  return 0;
+}
+
 int radik_plan_set_timing(radik_plan* p, int enable) {
   clear_error();
   if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
@@ -323,7 +338,7 @@ int radik_plan_execute(radik_plan* p, float* val_out, void* idx_out, const float
       sp.k = p->ks[(size_t)t];
       sp.flags = tflags;
     }
-    rc = stage_tasks(p->pipe, p->specs.data(), T, *f, s);
+    rc = stage_tasks(p->pipe, p->specs.data(), T, *f, p->opts, s);
     if (rc) return rc;
     rc = upload(p->pipe, s);
     if (rc) return rc;
@@ -426,7 +441,9 @@ int radik_diag_pass1(const float* val_in, const int64_t* task_lens, int64_t num_
     off += task_lens[t];
   }
   hipStream_t s = (hipStream_t)stream;
-  rc = stage_tasks(pipe, specs.data(), num_tasks, all, s);
+  LaunchOpts o;
+  o.predict = false;
+  rc = stage_tasks(pipe, specs.data(), num_tasks, all, o, s);
   if (rc) return rc;
   rc = upload(pipe, s);
   if (rc) return rc;

@@ -457,7 +457,7 @@ __device__ __forceinline__ void pass_b_segment(const TaskDesc& d, int64_t s, int
 template <int UNR>
 __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ bases, int ntasks,
-                                                   int64_t total, int64_t chunk,
+                                                   int64_t g_begin, int64_t total, int64_t chunk,
                                                    TaskCtl* __restrict__ ctl,
                                                    uint64_t* __restrict__ cand) {
   __shared__ uint64_t stage_w[BLOCK / 64][STAGE];
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
   const WaveStage sw{stage_w[threadIdx.x >> 6], &stage_n[0][threadIdx.x >> 6]};
   const WaveStage scd{stage_c[threadIdx.x >> 6], &stage_n[1][threadIdx.x >> 6]};
   __syncthreads();
-  int64_t g0 = (int64_t)blockIdx.x * chunk;
+  int64_t g0 = g_begin + (int64_t)blockIdx.x * chunk;
   const int64_t g1 = min(g0 + chunk, total);
   if (g0 >= g1) return;
   int t = find_task(bases, ntasks, g0);
@@ -931,8 +931,8 @@ __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ s
 #define RADIK_INST(U)                                                                                   \
   template __global__ void k_hist<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, uint32_t*, \
                                      TaskCtl*, uint64_t*);                                              \
-  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, TaskCtl*, \
-                                        uint64_t*);
+  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, int64_t,   \
+                                        TaskCtl*, uint64_t*);
 RADIK_INST(1)
 RADIK_INST(2)
 RADIK_INST(4)

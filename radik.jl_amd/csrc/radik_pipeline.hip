@@ -67,12 +67,15 @@ int dev_facts(DevFacts** out) {
     }
     f.lds_per_block = budget;
     const size_t stat = 20 * 1024;  // static LDS of k_refine / k_small (histogram + scratch)
-    f.refine_cap = (uint32_t)((budget - stat) / 8);
+    f.refine_cap_max = (uint32_t)((budget - stat) / 8);
+    f.small_cap_max = (uint32_t)((budget - stat) / 16);
+    // Defaults keep k_refine and k_small at two or more blocks per CU.
+    f.refine_cap = std::min<uint32_t>(4096, f.refine_cap_max);
+    f.small_cap = std::min<uint32_t>(2048, f.small_cap_max);
     // register-blocked sort: up to 64 per thread, LDS scratch of a power of two
     uint32_t sc = BLOCK;
     while (sc * 2 <= (uint32_t)(budget / 8) && sc * 2 <= (uint32_t)BLOCK * 64) sc *= 2;
     f.sort_cap = sc;
-    f.small_cap = (uint32_t)((budget - stat) / 16);
     f.device = dev;
   }
   *out = &f;
@@ -191,21 +194,35 @@ static int64_t sample_groups(int64_t len, int64_t k) {
 
 // Short tasks run whole in one block (k_small); small_cap 0 routes every
 // task through the streaming passes.
-static inline bool is_short(int64_t len, const DevFacts& f) {
-  return f.small_cap > 0 && len <= (int64_t)f.small_cap;
+static inline bool is_short(int64_t len, uint32_t small_cap) {
+  return small_cap > 0 && len <= (int64_t)small_cap;
 }
 
-int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s) {
-  int64_t nl = 0, ns = 0, tot = 0;
+// Long tasks are laid out predicted first, then the ones pass B always
+// re-reads, so pass B can cover just the latter's elements densely.
+int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
+                hipStream_t s) {
+  const uint32_t scap = eff_small_cap(f, o);
+  int64_t nl = 0, ns = 0, tot = 0, tot_pred = 0;
   for (int64_t t = 0; t < n; ++t) {
-    if (is_short(tasks[t].len, f)) ++ns;
-    else { ++nl; tot += tasks[t].len; }
+    if (is_short(tasks[t].len, scap)) {
+      ++ns;
+    } else {
+      ++nl;
+      tot += tasks[t].len;
+      if (o.predict && sample_groups(tasks[t].len, tasks[t].k)) tot_pred += tasks[t].len;
+    }
   }
   int rc = p.reserve(nl, ns, tot, s);
   if (rc) return rc;
   int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0;
+  // pass 0: predicted long tasks; pass 1: short tasks and the other long tasks
+  for (int pass = 0; pass < 2; ++pass)
   for (int64_t t = 0; t < n; ++t) {
     const TaskSpec& ts = tasks[t];
+    const bool shrt = is_short(ts.len, scap);
+    const bool pred = o.predict && !shrt && sample_groups(ts.len, ts.k) > 0;
+    if ((pass == 0) != pred) continue;
     TaskDesc d;
     d.in = ts.in;
     d.idx_in = ts.idx_in;
@@ -214,7 +231,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     d.len = ts.len;
     d.k = (int32_t)ts.k;
     d.flags = ts.flags;
-    if (is_short(ts.len, f)) {
+    if (shrt) {
       d.base = 0;
       p.h_small[is++] = d;
     } else {
@@ -225,7 +242,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
       if (m > max_m) max_m = m;
       p.h_large[il++] = d;
       base += ts.len;
-      groups += sample_groups(ts.len, ts.k);
+      groups += pred ? sample_groups(ts.len, ts.k) : 0;
     }
   }
   p.h_bases[il] = base;
@@ -233,6 +250,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.n_large = nl;
   p.n_small = ns;
   p.total = tot;
+  p.total_pred = tot_pred;
   p.total_groups = groups;
   p.max_m = (uint32_t)max_m;
   return 0;
@@ -260,18 +278,20 @@ static void launch_stream_kernels_hist(const Pipeline& p, int64_t chunk, int gri
 }
 
 template <int UNR>
-static void launch_stream_kernels_compact(const Pipeline& p, int64_t chunk, int grid, hipStream_t s) {
+static void launch_stream_kernels_compact(const Pipeline& p, int64_t g_begin, int64_t g_end, int64_t chunk,
+                                          int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_compact<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     p.total, chunk, p.d_ctl, p.d_cand);
+                     g_begin, g_end, chunk, p.d_ctl, p.d_cand);
 }
 
-static int64_t stream_chunk(const Pipeline& p, const DevFacts& f, const LaunchOpts& o, int* grid) {
+// Equal chunks of `total` elements, about blocks_per_cu blocks per CU.
+static int64_t stream_chunk(int64_t total, const DevFacts& f, const LaunchOpts& o, int* grid) {
   const int64_t want_blocks = (int64_t)f.cus * o.blocks_per_cu;
-  int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
+  int64_t chunk = (total + want_blocks - 1) / want_blocks;
   const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
   if (chunk < 4 * gran) chunk = 4 * gran;
   chunk = (chunk + gran - 1) / gran * gran;
-  *grid = (int)std::max<int64_t>(1, (p.total + chunk - 1) / chunk);
+  *grid = (int)std::max<int64_t>(1, (total + chunk - 1) / chunk);
   return chunk;
 }
 
@@ -287,7 +307,7 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
   if ((rc = mark())) return rc;  // start
   if (p.n_large) {
     int grid = 0;
-    const int64_t chunk = stream_chunk(p, f, o, &grid);
+    const int64_t chunk = stream_chunk(p.total, f, o, &grid);
     if (p.total_groups > 0) {
       const int sgrid = (int)((p.total_groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK);
       hipLaunchKernelGGL(k_sample, dim3(sgrid), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
@@ -309,15 +329,25 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
     hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    switch (o.unr) {
-      case 1: launch_stream_kernels_compact<1>(p, chunk, grid, s); break;
-      case 4: launch_stream_kernels_compact<4>(p, chunk, grid, s); break;
-      default: launch_stream_kernels_compact<2>(p, chunk, grid, s); break;
+    // Pass B: the tasks without a prediction (all of them re-read), then the
+    // predicted ones (blocks skip tasks whose spill covered the bin).
+    for (int part = 0; part < 2; ++part) {
+      const int64_t gb = part == 0 ? p.total_pred : 0;
+      const int64_t ge = part == 0 ? p.total : p.total_pred;
+      if (ge <= gb) continue;
+      int g2 = 0;
+      const int64_t c2 = stream_chunk(ge - gb, f, o, &g2);
+      switch (o.unr) {
+        case 1: launch_stream_kernels_compact<1>(p, gb, ge, c2, g2, s); break;
+        case 4: launch_stream_kernels_compact<4>(p, gb, ge, c2, g2, s); break;
+        default: launch_stream_kernels_compact<2>(p, gb, ge, c2, g2, s); break;
+      }
+      RADIK_HIP(hipGetLastError());
     }
-    RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)f.refine_cap * 8, s,
-                       p.d_large, p.d_ctl, p.d_cand, f.refine_cap);
+    const uint32_t rcap = eff_refine_cap(f, o);
+    hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)rcap * 8, s, p.d_large,
+                       p.d_ctl, p.d_cand, rcap);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
     {
@@ -343,8 +373,8 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
   }
   if ((rc = mark())) return rc;  // end of sort
   if (p.n_small) {
-    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)f.small_cap * 16, s,
-                       p.d_small, f.small_cap);
+    const uint32_t scap = eff_small_cap(f, o);
+    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, s, p.d_small, scap);
     RADIK_HIP(hipGetLastError());
   }
   if ((rc = mark())) return rc;  // end of small
@@ -377,13 +407,9 @@ int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel
   if (!p.n_large) return 0;
   LaunchOpts o;
   int grid = 0;
-  const int64_t chunk = stream_chunk(p, f, o, &grid);
+  const int64_t chunk = stream_chunk(p.total, f, o, &grid);
   // Control blocks without a prediction: pass A only counts.
-  RADIK_HIP(hipStreamSynchronize(s));  // the pinned arrays may still feed the upload
-  p.total_groups = 0;
-  for (int64_t t = 0; t <= p.n_large; ++t) p.h_sbases[t] = 0;
-  RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-  RADIK_HIP(hipStreamSynchronize(s));
+  // staged without prediction: k_predict only initialises the control blocks
   hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
                      p.d_ctl);
   RADIK_HIP(hipGetLastError());

@@ -134,7 +134,7 @@ static void dispatch_loop(radik_server* s) {
       slot->tickets.push_back(r.ticket);
     }
     LaunchOpts o;
-    int rc = stage_tasks(slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, slot->stream);
+    int rc = stage_tasks(slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, o, slot->stream);
     if (!rc) rc = upload(slot->pipe, slot->stream);
     if (!rc) rc = enqueue(slot->pipe, *f, o, slot->stream, nullptr);
     if (!rc && hipEventRecord(slot->done, slot->stream) != hipSuccess) rc = RADIK_EHIP;

@@ -54,6 +54,7 @@ SIGNATURES = {
     "radik_plan_create": (_INT, [ctypes.POINTER(_P), _I64P, _I64P, _I64, _U32, _INT, _P]),
     "radik_plan_execute": (_INT, [_P, _P, _P, _P, _P, _P]),
     "radik_plan_set_graph": (_INT, [_P, _INT]),
+    "radik_plan_set_options": (_INT, [_P, _INT, _INT, _INT, _INT, _INT]),
     "radik_plan_set_timing": (_INT, [_P, _INT]),
     "radik_plan_timing": (_INT, [_P, ctypes.POINTER(ctypes.c_double), _I64P]),
     "radik_plan_destroy": (_INT, [_P]),
