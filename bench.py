@@ -97,10 +97,10 @@ def time_plan(radik, plan, vo, io, d_x, steps, warmup):
     return e0.elapsed_ms(e1) / steps, wall / steps * 1e3
 
 
-def phase_times(radik, plan, vo, io, d_x, steps):
+def phase_times(radik, plan, vo, io, d_x, steps, d_i=None):
     plan.set_timing(True)
     for _ in range(steps):
-        plan.execute(vo, io, d_x)
+        plan.execute(vo, io, d_x, d_i)
     radik.synchronize()
     ms, n = plan.timing()
     plan.set_timing(False)
@@ -168,7 +168,7 @@ def secondary(radik, O, warmup, steps):
         e1.record()
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / steps
-        ph, cnt = phase_times(radik, p, vo, io, d, steps) if k == 1024 else (None, 0)
+        ph, cnt = phase_times(radik, p, vo, io, d, steps, di)
         B = 8.0 * n + 8.0 * k
         r = {"ms": round(ms, 4), "Gelem_s": round(n / (ms * 1e-3) / 1e9, 1),
              "path_hbm_frac": round(B / (ms * 1e-3) / HBM_PEAK, 4)}
@@ -184,6 +184,38 @@ def secondary(radik, O, warmup, steps):
     return out
 
 
+def tune_all(radik, out_path):
+    """Autotunes the plan options on the benchmark workloads; writes the table."""
+    from radik import DeviceArray, autotune
+    table = autotune.load_table()
+    x, lens = uneven_batch()
+    work = [("uneven", x, lens, 256, None)]
+    rng = np.random.default_rng(2828)
+    n = 1 << 28
+    x3 = rng.standard_normal(n, dtype=np.float32)
+    for k in (16, 1024, 4096, 16384):
+        work.append((f"single2^28_k{k}", x3, np.array([n]), k, None))
+    u = np.random.default_rng(1001).random(1000 * 1000, dtype=np.float32)
+    work.append(("uniform1000x1000", u, np.array([1000] * 1000), 100, None))
+    rows = np.random.default_rng(5150).integers(1, 257, 8)
+    for r in (1, 16, 128):
+        xs = (np.random.default_rng(int(r)).standard_normal(r * 128256) * 3).astype(np.float32)
+        work.append((f"rows{r}x128256", xs, np.array([128256] * r), 50, None))
+    for name, xx, ll, k, _ in work:
+        d = DeviceArray.from_host(xx)
+        T = len(ll)
+        p = radik.Plan(ll, k, tuned=False)
+        vo = DeviceArray.zeros((k, T), np.float32)
+        io = DeviceArray.zeros((k, T), np.int32)
+        best, ms, trials = autotune.tune(p, vo, io, d, steps=5, log=log)
+        autotune.record(table, p, best, ms)
+        log(f"tuned {name}: {best} {ms:.4f} ms (default {trials[0][1]:.4f} ms)")
+        del d
+    autotune.save_table(table, out_path)
+    autotune.save_table(table)
+    print(json.dumps(table), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,7 +224,15 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="time the batched plan only (profiling runs)")
+    ap.add_argument("--tune", default="",
+                    help="run the autotuner on the bench workloads and write the table to this path")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run the plain copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.only_batched:
+        args.no_secondary = args.no_cpu = args.no_plain = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the radix top-k path runs on a single MI355X here "
               "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
@@ -204,6 +244,18 @@ def main():
 
     info = radik.device_info()
     log("device", info, radik.lds_caps())
+    if args.copy_bytes:
+        from radik import _lib
+        nb = args.copy_bytes // 16 * 16
+        src = DeviceArray.zeros(nb // 4, np.float32)
+        dst = DeviceArray.zeros(nb // 4, np.float32)
+        for _ in range(5):
+            _lib.check(_lib.load().radik_diag_copy(dst.ptr, src.ptr, nb, None))
+        del src, dst
+        log(f"copy kernel: 5 x {nb} bytes")
+    if args.tune:
+        tune_all(radik, args.tune)
+        return
     t0 = time.perf_counter()
     x, lens = uneven_batch()
     N = int(lens.sum())

@@ -87,12 +87,19 @@ int radik_plan_execute(radik_plan *plan, float *val_out, void *idx_out, const fl
                        const void *idx_in, void *stream);
 /* use_graph = 0 launches the kernels eagerly (same results). */
 int radik_plan_set_graph(radik_plan *plan, int use_graph);
+/* Tuning knobs (results do not depend on them): vload_bytes 16/32/64 (bytes
+ * in flight per lane per step of the streaming passes, 0 = keep), streaming
+ * blocks per CU (0 = keep), longest task run whole in one block, candidates
+ * held in LDS by the refinement (0 = device defaults), and whether long tasks
+ * are sampled so one read of the input suffices. */
+int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu, int small_cap, int refine_cap,
+                           int predict);
 /* Per-kernel timing: when enabled, execute runs eagerly and records HIP events
  * around each kernel on the plan's stream; radik_plan_timing returns the
- * accumulated milliseconds per phase (hist, select, compact, refine, sort,
- * small) and the number of executes timed. */
+ * accumulated milliseconds of the 8 phases (sample, predict, hist, select,
+ * compact, refine, sort, small) and the number of executes timed. */
 int radik_plan_set_timing(radik_plan *plan, int enable);
-int radik_plan_timing(radik_plan *plan, double *ms6, int64_t *count);
+int radik_plan_timing(radik_plan *plan, double *ms8, int64_t *count);
 int radik_plan_destroy(radik_plan *plan);
 
 /* Serving engine (no reference counterpart): continuous batching of top-k
@@ -125,6 +132,9 @@ int radik_diag_pass1(const float *val_in, const int64_t *task_lens, int64_t num_
  * (counterpart of @warpreduce, src/utils.jl:84-100): out_wave[i] = inclusive
  * scan within i's wave, out_block[i] = inclusive scan within i's block of 256. */
 int radik_diag_scan(const uint32_t *in, uint32_t *out_wave, uint32_t *out_block, int64_t n, void *stream);
+/* A plain 16-byte streaming copy of `bytes` (multiple of 16): the known-size
+ * reference that scales the profiler's HBM traffic counters. */
+int radik_diag_copy(void *dst, const void *src, int64_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

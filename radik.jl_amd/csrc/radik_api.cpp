@@ -16,6 +16,7 @@ namespace radik {
 const char* last_error();
 int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStream_t s);
 int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s);
+int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 }  // namespace radik
 
 using namespace radik;
@@ -52,7 +53,7 @@ struct radik_plan {
   hipStream_t graph_stream = nullptr;
   // timing
   PhaseTimer timer;
-  double ms[6] = {0, 0, 0, 0, 0, 0};
+  double ms[NUM_PHASES] = {};
   int64_t timed = 0;
   ~radik_plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -143,7 +144,7 @@ static int run_tasks(Pipeline& pipe, std::vector<TaskSpec>& specs, const LaunchO
   DevFacts* f = nullptr;
   int rc = dev_facts(&f);
   if (rc) return rc;
-  rc = stage_tasks(pipe, specs.data(), (int64_t)specs.size(), *f, s);
+  rc = stage_tasks(pipe, specs.data(), (int64_t)specs.size(), *f, o, s);
   if (rc) return rc;
   rc = upload(pipe, s);
   if (rc) return rc;
@@ -275,6 +276,21 @@ int radik_plan_set_graph(radik_plan* p, int use_graph) {
   return 0;
 }
 
+int radik_plan_set_options(radik_plan* p, int vload_bytes, int blocks_per_cu, int small_cap, int refine_cap,
+                           int predict) {
+  clear_error();
+  if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
+  if (blocks_per_cu < 0 || blocks_per_cu > 64 || small_cap < 0 || refine_cap < 0)
+    return set_error(RADIK_EINVAL, "bad plan options");
+  if (vload_bytes > 0) p->opts = opts_for(p->flags | RADIK_WITHPACKING, vload_bytes);
+  if (blocks_per_cu > 0) p->opts.blocks_per_cu = blocks_per_cu;
+  p->opts.small_cap = (uint32_t)small_cap;
+  p->opts.refine_cap = (uint32_t)refine_cap;
+  p->opts.predict = predict != 0;
+  p->staged = false;  // re-stage (task order may change) and re-capture
+  return 0;
+}
+
 int radik_plan_set_timing(radik_plan* p, int enable) {
   clear_error();
   if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
@@ -286,11 +302,11 @@ int radik_plan_set_timing(radik_plan* p, int enable) {
   return 0;
 }
 
-int radik_plan_timing(radik_plan* p, double* ms6, int64_t* count) {
+int radik_plan_timing(radik_plan* p, double* ms8, int64_t* count) {
   clear_error();
   if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
-  if (ms6)
-    for (int i = 0; i < 6; ++i) ms6[i] = p->ms[i];
+  if (ms8)
+    for (int i = 0; i < NUM_PHASES; ++i) ms8[i] = p->ms[i];
   if (count) *count = p->timed;
   return 0;
 }
@@ -322,7 +338,7 @@ int radik_plan_execute(radik_plan* p, float* val_out, void* idx_out, const float
       sp.k = p->ks[(size_t)t];
       sp.flags = tflags;
     }
-    rc = stage_tasks(p->pipe, p->specs.data(), T, *f, s);
+    rc = stage_tasks(p->pipe, p->specs.data(), T, *f, p->opts, s);
     if (rc) return rc;
     rc = upload(p->pipe, s);
     if (rc) return rc;
@@ -335,8 +351,8 @@ int radik_plan_execute(radik_plan* p, float* val_out, void* idx_out, const float
   if (p->timer.on) {
     rc = enqueue(p->pipe, *f, p->opts, s, &p->timer);
     if (rc) return rc;
-    RADIK_HIP(hipEventSynchronize(p->timer.ev[6]));
-    for (int i = 0; i < 6; ++i) {
+    RADIK_HIP(hipEventSynchronize(p->timer.ev[NUM_PHASES]));
+    for (int i = 0; i < NUM_PHASES; ++i) {
       float ms = 0;
       RADIK_HIP(hipEventElapsedTime(&ms, p->timer.ev[i], p->timer.ev[i + 1]));
       p->ms[i] += ms;
@@ -425,13 +441,25 @@ int radik_diag_pass1(const float* val_in, const int64_t* task_lens, int64_t num_
     off += task_lens[t];
   }
   hipStream_t s = (hipStream_t)stream;
-  rc = stage_tasks(pipe, specs.data(), num_tasks, all, s);
+  LaunchOpts o;
+  o.predict = false;
+  rc = stage_tasks(pipe, specs.data(), num_tasks, all, o, s);
   if (rc) return rc;
   rc = upload(pipe, s);
   if (rc) return rc;
   rc = diag_pass1(pipe, all, hist_out, sel_out, s);
   if (rc) return rc;
   RADIK_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int radik_diag_copy(void* dst, const void* src, int64_t bytes, void* stream) {
+  clear_error();
+  if (bytes < 0 || (bytes > 0 && (!dst || !src)) || (bytes % 16) != 0)
+    return set_error(RADIK_EINVAL, "bytes must be a multiple of 16 with non-NULL buffers");
+  int rc = diag_copy(dst, src, bytes, (hipStream_t)stream);
+  if (rc) return rc;
+  RADIK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return 0;
 }
 

@@ -44,8 +44,11 @@ struct TaskCtl {
   uint32_t nwin;      // winners expected (= m - krem)
   uint32_t m;         // min(k, len): valid outputs
   uint32_t win_ctr;   // slot counter for winners (compaction)
-  uint32_t cand_ctr;  // slot counter for candidates (compaction)
-  uint32_t pad[8];
+  uint32_t cand_ctr;  // entries in the task's spill region
+  uint32_t b_pred;    // predicted bound: pass A spills first digits <= b_pred (NO_BIN: no spill)
+  uint32_t spill_all; // 1: the spill holds winners and candidates; 0: candidates only
+  uint32_t fallback;  // 1: pass B re-reads the task (prediction missing or too tight)
+  uint32_t pad[5];
 };
 static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 

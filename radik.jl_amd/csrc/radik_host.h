@@ -28,35 +28,50 @@ struct DevFacts {
   int device = -1;
   int cus = 256;
   size_t lds_per_block = 65536;
-  uint32_t small_cap = 2048;   // longest task routed to k_small
-  uint32_t refine_cap = 4096;  // candidates kept in LDS by k_refine
+  uint32_t small_cap = 2048;   // default: longest task routed to k_small
+  uint32_t refine_cap = 4096;  // default: candidates kept in LDS by k_refine
+  uint32_t small_cap_max = 2048, refine_cap_max = 4096;  // what the LDS budget allows
   uint32_t sort_cap = 4096;    // results sorted in LDS by k_sort
 };
 int dev_facts(DevFacts** out);
 
 // Launch-level options.
 struct LaunchOpts {
-  int unr = 2;            // float4 loads in flight per thread (vload bytes / 16)
-  int blocks_per_cu = 8;  // streaming grid: blocks per CU
+  int unr = 2;              // float4 loads in flight per thread (vload bytes / 16)
+  int blocks_per_cu = 8;    // streaming grid: blocks per CU
+  uint32_t small_cap = 0;   // longest task for k_small (0: device default)
+  uint32_t refine_cap = 0;  // candidates held in LDS by k_refine (0: device default)
+  bool predict = true;      // sample long tasks and spill in pass A
 };
+inline uint32_t eff_small_cap(const DevFacts& f, const LaunchOpts& o) {
+  return o.small_cap ? (o.small_cap < f.small_cap_max ? o.small_cap : f.small_cap_max) : f.small_cap;
+}
+inline uint32_t eff_refine_cap(const DevFacts& f, const LaunchOpts& o) {
+  return o.refine_cap ? (o.refine_cap < f.refine_cap_max ? o.refine_cap : f.refine_cap_max) : f.refine_cap;
+}
 
 // Device + pinned buffers of one pipeline instance.  Grow-only; the
 // histogram is kept all-zero between launches (k_select clears what it reads).
 struct Pipeline {
   // host copies (pinned)
   TaskDesc* h_large = nullptr;
-  int64_t* h_bases = nullptr;
+  int64_t* h_bases = nullptr;   // element prefix sums of the long tasks
+  int64_t* h_sbases = nullptr;  // sample-group prefix sums of the long tasks
   TaskDesc* h_small = nullptr;
   // device
   TaskDesc* d_large = nullptr;
   int64_t* d_bases = nullptr;
+  int64_t* d_sbases = nullptr;
   TaskDesc* d_small = nullptr;
   TaskCtl* d_ctl = nullptr;
-  uint32_t* d_hist = nullptr;
-  uint64_t* d_cand = nullptr;
+  uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
+  uint32_t* d_shist = nullptr;  // sample histograms, zero between launches
+  uint64_t* d_cand = nullptr;   // spill regions
   size_t cap_large = 0, cap_small = 0, cap_hist = 0, cap_cand = 0;
   // current content
-  int64_t n_large = 0, n_small = 0, total = 0;
+  int64_t n_large = 0, n_small = 0, total = 0, total_groups = 0;
+  int64_t total_pred = 0;  // elements of the predicted tasks, which come first
+  uint32_t max_m = 0;  // largest min(k, len) over the long tasks
   hipEvent_t upload_done = nullptr;  // pinned buffers reusable after this
   ~Pipeline();
   int reserve(int64_t n_large, int64_t n_small, int64_t total_large, hipStream_t s);
@@ -73,11 +88,14 @@ struct TaskSpec {
   int64_t k;
   uint32_t flags;
 };
-int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s);
+int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
+                hipStream_t s);
 
 // Phase timing hooks (events recorded between kernels when non-null).
+// Phases: sample, predict, hist, select, compact, refine, sort, small.
+constexpr int NUM_PHASES = 8;
 struct PhaseTimer {
-  hipEvent_t ev[7] = {};
+  hipEvent_t ev[NUM_PHASES + 1] = {};
   bool on = false;
 };
 

@@ -6,23 +6,31 @@
 // (rank-key bits [31:20], the reference's pass-1 digit, count_bin.jl:78-203,
 // algorithm.jl:221-224) is resolved over the raw input by two streaming passes:
 //
-//   k_hist     4096-bin histogram per task (LDS ds_add, one global atomic per
-//              non-zero bin per block)                       ~ count_bin_ex_kernel!
+//   k_sample   first-digit histogram of an evenly spread ~1/64 sample of each
+//              long task
+//   k_predict  per task: a first-digit bound b_pred that the task's m best
+//              keys stay under with high probability
+//   k_hist     pass A, the one full read: exact 4096-bin histogram per task
+//              (LDS ds_add, one global atomic per non-zero bin per block) and,
+//              in the same read, a spill of every key with digit <= b_pred
+//              (wave ballot + one atomic per wave)            ~ count_bin_ex_kernel!
 //   k_select   block scan of the 4096 bins -> winner bound, candidate bin,
-//              residual k                                    ~ select_bin_kernel!
-//   k_compact  second read: winners (digit < bin) go straight to the output
-//              slots, candidates (digit == bin) are spilled as composites;
-//              wave ballot + one atomic per wave              ~ select_candidate_ex_kernel!
-//   k_refine   per task: the remaining digits (rank bits [19:8], [7:0] as in
+//              residual k; checks the spill covers the bin  ~ select_bin_kernel!
+//   k_compact  pass B, only for tasks whose prediction was missing or too
+//              tight: re-reads them, winners (digit < bin) to the output slots,
+//              candidates (digit == bin) to the spill    ~ select_candidate_ex_kernel!
+//   k_refine   per task: emits the spilled winners, then resolves the
+//              remaining digits (rank bits [19:8], [7:0] as in
 //              algorithm.jl:247-299, then position digits for exact ties) over
-//              the spilled candidates, in LDS when they fit  ~ passes 2/3 + filter
+//              the candidates, in LDS when they fit          ~ passes 2/3 + filter
 //   k_sort     per task: bitonic sort of the k results by (output key,
 //              position), indices gathered                    ~ bitonic_sort!/merge_sort
 //   k_small    one block per short task: everything above in LDS, one launch.
 //
-// Streaming kernels (k_hist, k_compact) split the launch's concatenated element
-// space into equal chunks, one per block, so uneven task lengths balance at
-// element granularity; a chunk that crosses task boundaries flushes per task.
+// Streaming kernels (k_sample, k_hist, k_compact) split the launch's
+// concatenated element space into equal chunks, one per block, so uneven task
+// lengths balance at element granularity; a chunk that crosses task boundaries
+// flushes per task.
 
 #include "radik_device.h"
 
@@ -79,16 +87,228 @@ __device__ __forceinline__ void stream_range(const float* __restrict__ p, int64_
   for (int64_t i = a + (nvec << 2) + tid; i < e; i += BLOCK) f(p[i], (uint32_t)i);
 }
 
+// Rank key with the task's scaling decided at compile time.
+template <bool SCALE>
+__device__ __forceinline__ uint32_t rk_of(float x, float sc, uint32_t xm) {
+  return SCALE ? (fkey(scaled(x, sc)) ^ xm) : (fkey(x) ^ xm);
+}
+
+// Stores a result (value, position) into output slot `slot` of task d.
+__device__ __forceinline__ void put_result(const TaskDesc& d, uint32_t slot, float v, uint32_t pos) {
+  d.val_out[slot] = v;
+  if (d.flags & F_IDX64) reinterpret_cast<int64_t*>(d.idx_out)[slot] = (int64_t)pos;
+  else reinterpret_cast<int32_t*>(d.idx_out)[slot] = (int32_t)pos;
+}
+
+// Wave-aggregated append: every lane with `p` gets a distinct slot from
+// *ctr; one atomic per wave.  Must be reached by all active lanes together.
+__device__ __forceinline__ uint32_t wave_append(bool p, uint64_t mask, uint32_t* ctr) {
+  const uint64_t act = ballot(true);
+  uint32_t base = 0;
+  if (lanes_below(act) == 0) base = atomicAdd(ctr, (uint32_t)__builtin_popcountll(mask));
+  base = __builtin_amdgcn_readfirstlane(base);
+  return base + lanes_below(mask);
+}
+
+// Per-wave LDS staging of appended entries: lanes with `p` write their entry
+// into the wave's stage; when the stage could overflow on the next append
+// (or at the end of a segment) the wave reserves space for all of them with
+// one global atomic and copies them out.  Entries are 64-bit; flush() hands
+// them to a writer (index in the destination, entry).
+constexpr uint32_t STAGE = 512;  // entries per wave
+
+struct WaveStage {
+  uint64_t* buf;  // this wave's STAGE entries in LDS
+  uint32_t* n;    // this wave's fill, in LDS so lanes that sat out a divergent
+                  // append still read the current value
+};
+
+__device__ __forceinline__ void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename W>
+__device__ __forceinline__ void stage_flush(const WaveStage& st, uint32_t* ctr, W&& write) {
+  lds_wave_sync();
+  const uint32_t n = *st.n;
+  if (n == 0) return;
+  const uint64_t act = ballot(true);
+  const uint32_t nact = (uint32_t)__builtin_popcountll(act);
+  const uint32_t me = lanes_below(act);
+  uint32_t base = 0;
+  if (me == 0) base = atomicAdd(ctr, n);
+  base = __builtin_amdgcn_readfirstlane(base);
+  for (uint32_t i = me; i < n; i += nact) write(base + i, st.buf[i]);
+  lds_wave_sync();
+  if (me == 0) *st.n = 0;
+  lds_wave_sync();
+}
+
+template <typename W>
+__device__ __forceinline__ void stage_append(const WaveStage& st, bool p, uint64_t v, uint32_t* ctr, W&& write) {
+  const uint64_t mk = ballot(p);
+  if (!mk) return;
+  lds_wave_sync();
+  const uint32_t n0 = *st.n;
+  if (p) st.buf[n0 + lanes_below(mk)] = v;
+  const uint32_t n1 = n0 + (uint32_t)__builtin_popcountll(mk);
+  lds_wave_sync();
+  if (lanes_below(ballot(true)) == 0) *st.n = n1;
+  lds_wave_sync();
+  if (n1 > STAGE - 64) stage_flush(st, ctr, write);
+}
+
+// LDS histogram increment with a wave-level pre-merge: when every active
+// lane hits the same bin (ties), one lane adds the count.
+__device__ __forceinline__ void hist_add(uint32_t* lh, uint32_t bin) {
+  const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
+  const uint64_t act = ballot(true);
+  if (ballot(bin == b0) == act) {
+    if (lanes_below(act) == 0) atomicAdd(&lh[b0], (uint32_t)__builtin_popcountll(act));
+  } else {
+    atomicAdd(&lh[bin], 1u);
+  }
+}
+
+// Largest rank key whose first digit is <= b (b < 4096).
+__device__ __forceinline__ uint32_t digit_bound(uint32_t b) {
+  return (uint32_t)((((uint64_t)b + 1) << 20) - 1);
+}
+
+// Adds a block's LDS histogram into a task's global one and clears it.
+__device__ __forceinline__ void flush_hist(uint32_t* __restrict__ lh, uint32_t* __restrict__ h) {
+  for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) {
+    const uint32_t c = lh[i];
+    if (c) {
+      atomicAdd(&h[i], c);
+      lh[i] = 0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Pass A: first-digit histogram.
+// Sampling: a predicted tasks owns G groups of SAMPLE_GROUP consecutive
+// elements spread evenly over it (group j starts at j * len / G).  Blocks take
+// equal runs of groups of the launch; per task the first-digit histogram of
+// the sample goes to shist.
 // ---------------------------------------------------------------------------
+constexpr int SAMPLE_GROUP = BLOCK;
+constexpr int SAMPLE_GROUPS_PER_BLOCK = 16;
+
+__global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ tasks,
+                                                  const int64_t* __restrict__ sbases, int ntasks,
+                                                  int64_t total_groups, uint32_t* __restrict__ shist) {
+  __shared__ uint32_t lh[HIST_BINS];
+  for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
+  int64_t g0 = (int64_t)blockIdx.x * SAMPLE_GROUPS_PER_BLOCK;
+  const int64_t g1 = min(g0 + (int64_t)SAMPLE_GROUPS_PER_BLOCK, total_groups);
+  if (g0 >= g1) return;
+  int t = find_task(sbases, ntasks, g0);
+  __syncthreads();
+  while (true) {
+    const TaskDesc d = tasks[t];
+    const int64_t tb = sbases[t];
+    const int64_t G = sbases[t + 1] - tb;
+    const int64_t se = min(g1, tb + G);
+    const float sc = task_scaler(d);
+    const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
+    for (int64_t g = g0; g < se; ++g) {
+      const int64_t i = (g - tb) * d.len / G + threadIdx.x;
+      if (i < d.len) atomicAdd(&lh[rank_key(d.in[i], sc, xm) >> 20], 1u);
+    }
+    __syncthreads();
+    flush_hist(lh, shist + (size_t)t * HIST_BINS);
+    if (se >= g1) break;
+    g0 = se;
+    ++t;
+    while (t + 1 < ntasks && sbases[t + 1] <= g0) ++t;  // tasks without samples
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Prediction, one block per task; initialises the task's control block.
+// From the sample histogram it picks the smallest first digit b_pred such
+// that the sample holds at least q = f*m + 4*sqrt(f*m) + 4 keys with digit
+// <= b_pred (f = sampled fraction, m = min(k, len)), so the task's m best
+// keys lie at or below b_pred unless the sample is off by four standard
+// deviations; a miss only costs the fallback re-read.  Clears the sample bins.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ tasks,
+                                                   const int64_t* __restrict__ sbases,
+                                                   uint32_t* __restrict__ shist,
+                                                   TaskCtl* __restrict__ ctl) {
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t res[3];
+  const int t = blockIdx.x;
+  const TaskDesc d = tasks[t];
+  const int64_t G = sbases[t + 1] - sbases[t];
+  const int64_t m = d.k < d.len ? (int64_t)d.k : d.len;
+  if (threadIdx.x == 0) res[0] = NO_BIN;
+  if (G > 0) {
+    uint4* __restrict__ h4 = reinterpret_cast<uint4*>(shist + (size_t)t * HIST_BINS) + threadIdx.x * 4;
+    uint32_t c[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint4 v = h4[q];
+      c[4 * q + 0] = v.x;
+      c[4 * q + 1] = v.y;
+      c[4 * q + 2] = v.z;
+      c[4 * q + 3] = v.w;
+      h4[q] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    const float S = (float)(G * SAMPLE_GROUP);
+    const float mu = S / (float)d.len * (float)m;
+    const float q = mu + 4.0f * sqrtf(mu) + 4.0f;
+    const uint32_t need = (uint32_t)ceilf(q);
+    if (m > 0 && (float)need <= S) block_find_bin<16>(c, need, wsum, res);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    TaskCtl o = {};
+    o.b_win = 0;
+    o.b_cand = NO_BIN;
+    o.b_pred = (res[0] < (uint32_t)HIST_BINS) ? res[0] : NO_BIN;
+    ctl[t] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass A: exact first-digit histogram of every task; for predicted tasks the
+// same read also spills every key with first digit <= b_pred (composite
+// rank << 32 | position) into the task's spill region.
+// ---------------------------------------------------------------------------
+template <int UNR, bool SCALE, bool SPILL>
+__device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int64_t e, float sc, uint32_t xm,
+                                               uint32_t smax, uint32_t* lh, uint32_t* ctr,
+                                               uint64_t* __restrict__ spill, const WaveStage& st) {
+  const uint32_t slen = (uint32_t)d.len;
+  auto write = [&](uint32_t slot, uint64_t v) {
+    if (slot < slen) spill[slot] = v;
+  };
+  stream_range<UNR>(d.in, s, e, [&](float x, uint32_t pos) {
+    const uint32_t rk = rk_of<SCALE>(x, sc, xm);
+    atomicAdd(&lh[rk >> 20], 1u);
+    if (SPILL) stage_append(st, rk <= smax, ((uint64_t)rk << 32) | pos, ctr, write);
+  });
+  if (SPILL) stage_flush(st, ctr, write);
+}
+
 template <int UNR>
 __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tasks,
                                                 const int64_t* __restrict__ bases, int ntasks,
                                                 int64_t total, int64_t chunk,
-                                                uint32_t* __restrict__ hist) {
+                                                uint32_t* __restrict__ hist,
+                                                TaskCtl* __restrict__ ctl,
+                                                uint64_t* __restrict__ cand) {
   __shared__ uint32_t lh[HIST_BINS];
+  __shared__ uint64_t stage_buf[BLOCK / 64][STAGE];
+  __shared__ uint32_t stage_n[BLOCK / 64];
   for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
+  if (threadIdx.x < BLOCK / 64) stage_n[threadIdx.x] = 0;
+  const WaveStage st{stage_buf[threadIdx.x >> 6], &stage_n[threadIdx.x >> 6]};
   int64_t g0 = (int64_t)blockIdx.x * chunk;
   const int64_t g1 = min(g0 + chunk, total);
   if (g0 >= g1) return;
@@ -97,22 +317,24 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
   while (true) {
     const TaskDesc d = tasks[t];
     const int64_t tb = bases[t];
-    const int64_t te = tb + d.len;
-    const int64_t se = min(g1, te);
+    const int64_t se = min(g1, tb + d.len);
     const float sc = task_scaler(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-    stream_range<UNR>(d.in, g0 - tb, se - tb, [&](float x, uint32_t) {
-      atomicAdd(&lh[rank_key(x, sc, xm) >> 20], 1u);
-    });
-    __syncthreads();
-    uint32_t* __restrict__ h = hist + (size_t)t * HIST_BINS;
-    for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) {
-      const uint32_t c = lh[i];
-      if (c) {
-        atomicAdd(&h[i], c);
-        lh[i] = 0;
-      }
+    const uint32_t bp = ctl[t].b_pred;
+    const bool scale = (d.flags & F_WITHSCALE) != 0;
+    uint32_t* ctr = &ctl[t].cand_ctr;
+    uint64_t* spill = cand + d.base;
+    const int64_t s = g0 - tb, e = se - tb;
+    if (bp == NO_BIN) {
+      if (scale) pass_a_segment<UNR, true, false>(d, s, e, sc, xm, 0u, lh, ctr, spill, st);
+      else pass_a_segment<UNR, false, false>(d, s, e, sc, xm, 0u, lh, ctr, spill, st);
+    } else {
+      const uint32_t smax = digit_bound(bp);
+      if (scale) pass_a_segment<UNR, true, true>(d, s, e, sc, xm, smax, lh, ctr, spill, st);
+      else pass_a_segment<UNR, false, true>(d, s, e, sc, xm, smax, lh, ctr, spill, st);
     }
+    __syncthreads();
+    flush_hist(lh, hist + (size_t)t * HIST_BINS);
     if (se >= g1) break;
     g0 = se;
     ++t;
@@ -123,7 +345,9 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
 
 // ---------------------------------------------------------------------------
 // First-digit selection, one block per task.  Reads and clears the task's
-// histogram so the buffer is zero again for the next launch.
+// histogram so the buffer is zero again for the next launch.  Decides whether
+// the spill of pass A already holds every key the task needs (first digit <=
+// the selected bin) or pass B must re-read the task.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ tasks,
                                                   uint32_t* __restrict__ hist,
@@ -148,9 +372,11 @@ __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ t
   if (threadIdx.x == 0) res[0] = NO_BIN;  // published by the barriers inside the scan
   if (m > 0 && m < len) block_find_bin<16>(c, (uint32_t)m, wsum, res);
   if (threadIdx.x == 0) {
+    const TaskCtl prev = ctl[t];
     TaskCtl o = {};
     o.b_cand = NO_BIN;
     o.krem = 0;
+    o.b_pred = prev.b_pred;
     if (m <= 0) {
       o.b_win = 0;
     } else if (m >= len) {
@@ -176,73 +402,88 @@ __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ t
     o.m = (uint32_t)(m > 0 ? m : 0);
     o.nwin = o.m - o.krem;
     o.win_ctr = 0;
-    o.cand_ctr = 0;
+    // Highest first digit the task needs: the candidate bin, or the last
+    // winner bin when the threshold bin is taken whole.
+    const uint32_t need = (o.b_cand != NO_BIN) ? o.b_cand : (o.b_win - 1);
+    if (o.m == 0) {
+      o.fallback = 0;
+      o.spill_all = 1;
+      o.cand_ctr = 0;
+    } else if (prev.b_pred == NO_BIN || need > prev.b_pred || prev.cand_ctr > (uint32_t)len) {
+      o.fallback = 1;
+      o.spill_all = 0;
+      o.cand_ctr = 0;
+    } else {
+      o.fallback = 0;
+      o.spill_all = 1;
+      o.cand_ctr = prev.cand_ctr;
+    }
     ctl[t] = o;
   }
 }
 
-// Stores a result (value, position) into output slot `slot` of task d.
-__device__ __forceinline__ void put_result(const TaskDesc& d, uint32_t slot, float v, uint32_t pos) {
-  d.val_out[slot] = v;
-  if (d.flags & F_IDX64) reinterpret_cast<int64_t*>(d.idx_out)[slot] = (int64_t)pos;
-  else reinterpret_cast<int32_t*>(d.idx_out)[slot] = (int32_t)pos;
+// ---------------------------------------------------------------------------
+// Pass B (only tasks whose prediction was missing or too tight): winners
+// (digit < bin) go to the output slots [0, nwin) as staging (value,
+// position); candidates (digit == bin) to the task's spill region.
+// ---------------------------------------------------------------------------
+template <int UNR, bool SCALE>
+__device__ __forceinline__ void pass_b_segment(const TaskDesc& d, int64_t s, int64_t e, float sc, uint32_t xm,
+                                               const TaskCtl& c, uint32_t* wctr, uint32_t* cctr,
+                                               uint64_t* __restrict__ cb, const WaveStage& sw,
+                                               const WaveStage& scd) {
+  const uint32_t bw = c.b_win, bc = c.b_cand, nwin = c.nwin;
+  const uint32_t rmax = digit_bound(bc != NO_BIN ? bc : bw - 1);
+  const uint32_t clen = (uint32_t)d.len;
+  auto write_win = [&](uint32_t slot, uint64_t v) {
+    if (slot < nwin) put_result(d, slot, __uint_as_float((uint32_t)(v >> 32)), (uint32_t)v);
+  };
+  auto write_cand = [&](uint32_t slot, uint64_t v) {
+    if (slot < clen) cb[slot] = v;
+  };
+  stream_range<UNR>(d.in, s, e, [&](float x, uint32_t pos) {
+    const uint32_t rk = rk_of<SCALE>(x, sc, xm);
+    const bool hit = rk <= rmax;
+    if (ballot(hit)) {
+      const uint32_t dg = rk >> 20;
+      stage_append(sw, hit && dg < bw, ((uint64_t)__float_as_uint(x) << 32) | pos, wctr, write_win);
+      stage_append(scd, hit && dg == bc, ((uint64_t)rk << 32) | pos, cctr, write_cand);
+    }
+  });
+  stage_flush(sw, wctr, write_win);
+  stage_flush(scd, cctr, write_cand);
 }
 
-// Wave-aggregated append: every lane with `p` gets a distinct slot from
-// *ctr; one atomic per wave.  Must be reached by all active lanes together.
-__device__ __forceinline__ uint32_t wave_append(bool p, uint64_t mask, uint32_t* ctr) {
-  const uint64_t act = ballot(true);
-  uint32_t base = 0;
-  if (lanes_below(act) == 0) base = atomicAdd(ctr, (uint32_t)__builtin_popcountll(mask));
-  base = __builtin_amdgcn_readfirstlane(base);
-  return base + lanes_below(mask);
-}
-
-// ---------------------------------------------------------------------------
-// Pass B: compaction.  Winners go to the output slots [0, nwin) as staging
-// (value, position); candidates to the task's region of the spill buffer.
-// ---------------------------------------------------------------------------
 template <int UNR>
 __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ bases, int ntasks,
-                                                   int64_t total, int64_t chunk,
+                                                   int64_t g_begin, int64_t total, int64_t chunk,
                                                    TaskCtl* __restrict__ ctl,
                                                    uint64_t* __restrict__ cand) {
-  int64_t g0 = (int64_t)blockIdx.x * chunk;
+  __shared__ uint64_t stage_w[BLOCK / 64][STAGE];
+  __shared__ uint64_t stage_c[BLOCK / 64][STAGE];
+  __shared__ uint32_t stage_n[2][BLOCK / 64];
+  if (threadIdx.x < 2 * (BLOCK / 64)) stage_n[threadIdx.x / (BLOCK / 64)][threadIdx.x % (BLOCK / 64)] = 0;
+  const WaveStage sw{stage_w[threadIdx.x >> 6], &stage_n[0][threadIdx.x >> 6]};
+  const WaveStage scd{stage_c[threadIdx.x >> 6], &stage_n[1][threadIdx.x >> 6]};
+  __syncthreads();
+  int64_t g0 = g_begin + (int64_t)blockIdx.x * chunk;
   const int64_t g1 = min(g0 + chunk, total);
   if (g0 >= g1) return;
   int t = find_task(bases, ntasks, g0);
   while (true) {
     const TaskDesc d = tasks[t];
     const int64_t tb = bases[t];
-    const int64_t te = tb + d.len;
-    const int64_t se = min(g1, te);
-    const uint32_t bw = ctl[t].b_win;
-    const uint32_t bc = ctl[t].b_cand;
-    const uint32_t nwin = ctl[t].nwin;
-    if (bw > 0 || bc != NO_BIN) {
+    const int64_t se = min(g1, tb + d.len);
+    const TaskCtl c = ctl[t];
+    if (c.fallback && c.m > 0) {
       const float sc = task_scaler(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
       uint32_t* wctr = &ctl[t].win_ctr;
       uint32_t* cctr = &ctl[t].cand_ctr;
-      uint64_t* __restrict__ cb = cand + d.base;
-      const uint32_t clen = (uint32_t)d.len;
-      stream_range<UNR>(d.in, g0 - tb, se - tb, [&](float x, uint32_t pos) {
-        const uint32_t rk = rank_key(x, sc, xm);
-        const uint32_t dg = rk >> 20;
-        const bool w = dg < bw;
-        const bool c = dg == bc;
-        const uint64_t mw = ballot(w);
-        const uint64_t mc = ballot(c);
-        if (mw) {
-          const uint32_t slot = wave_append(w, mw, wctr);
-          if (w && slot < nwin) put_result(d, slot, x, pos);
-        }
-        if (mc) {
-          const uint32_t slot = wave_append(c, mc, cctr);
-          if (c && slot < clen) cb[slot] = ((uint64_t)rk << 32) | pos;
-        }
-      });
+      uint64_t* cb = cand + d.base;
+      if (d.flags & F_WITHSCALE) pass_b_segment<UNR, true>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
+      else pass_b_segment<UNR, false>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
     }
     if (se >= g1) break;
     g0 = se;
@@ -278,7 +519,7 @@ __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint6
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
       const uint64_t x = src[i];
       const bool match = (hs >= 64) ? true : ((x >> hs) == P);
-      if (match) atomicAdd(&lh[(uint32_t)(x >> s) & (nb - 1)], 1u);
+      if (match) hist_add(lh, (uint32_t)(x >> s) & (nb - 1));
     }
     __syncthreads();
     if (w == 12) {
@@ -302,7 +543,9 @@ __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint6
 }
 
 // ---------------------------------------------------------------------------
-// Refinement of the spilled candidates, one block per task.
+// Refinement, one block per task: emits the winners held in the spill (pass A
+// spills), then resolves the candidates of the threshold bin digit by digit
+// (in LDS when they fit) and emits the chosen ones.
 // ---------------------------------------------------------------------------
 struct LdsSrc {
   const uint64_t* p;
@@ -313,22 +556,28 @@ struct GlobalSrc {
   __device__ uint64_t operator[](uint32_t i) const { return p[i]; }
 };
 
+// Value of a spilled composite: the original element.
+__device__ __forceinline__ float spilled_value(const TaskDesc& d, uint64_t x, uint32_t& pos) {
+  pos = (uint32_t)x;
+  if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
+  if (d.flags & F_WITHSCALE) return d.in[pos];
+  const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
+  return fkey_inv((uint32_t)(x >> 32) ^ xm);
+}
+
 template <typename Src>
 __device__ __forceinline__ void refine_emit(const TaskDesc& d, const TaskCtl& c, Src src, uint32_t n,
                                             uint32_t* lh, uint32_t* wsum, uint32_t* res, uint32_t* ectr) {
   uint64_t P = c.b_cand;
   const int s = select_digits(src, n, 52, P, c.krem, lh, wsum, res);
-  const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-  const bool scale = (d.flags & F_WITHSCALE) != 0;
   const uint32_t lim = c.m - c.nwin;
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     const uint64_t x = src[i];
-    if ((x >> s) <= P) {
+    if ((x >> 52) == c.b_cand && (x >> s) <= P) {
       const uint32_t slot = atomicAdd(ectr, 1u);
       if (slot < lim) {
-        uint32_t pos = (uint32_t)x;
-        if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
-        const float v = scale ? d.in[pos] : fkey_inv((uint32_t)(x >> 32) ^ xm);
+        uint32_t pos;
+        const float v = spilled_value(d, x, pos);
         put_result(d, c.nwin + slot, v, pos);
       }
     }
@@ -343,21 +592,51 @@ __global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ t
   __shared__ uint32_t lh[HIST_BINS];
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
-  __shared__ uint32_t ectr;
+  __shared__ uint32_t ectr, wctr, nlds;
   const int t = blockIdx.x;
   const TaskCtl c = ctl[t];
-  if (c.krem == 0) return;
+  if (c.m == 0 || (!c.spill_all && c.krem == 0)) return;
   const TaskDesc d = tasks[t];
   uint32_t n = c.cand_ctr;
   if (n > (uint32_t)d.len) n = (uint32_t)d.len;
-  if (threadIdx.x == 0) ectr = 0;
+  if (threadIdx.x == 0) {
+    ectr = 0;
+    wctr = 0;
+    nlds = 0;
+  }
+  __syncthreads();
   const uint64_t* __restrict__ src = cand + d.base;
-  if (n <= lds_cap) {
+  if (c.spill_all) {
+    // Split the spill: winners to the output slots, candidates to LDS.
+    const bool to_lds = c.krem > 0 && c.ncand <= lds_cap;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+      const uint64_t x = src[i];
+      const uint32_t dg = (uint32_t)(x >> 52);
+      if (dg < c.b_win) {
+        const uint32_t slot = atomicAdd(&wctr, 1u);
+        if (slot < c.nwin) {
+          uint32_t pos;
+          const float v = spilled_value(d, x, pos);
+          put_result(d, slot, v, pos);
+        }
+      } else if (to_lds && dg == c.b_cand) {
+        const uint32_t j = atomicAdd(&nlds, 1u);
+        if (j < lds_cap) cbuf[j] = x;
+      }
+    }
+    __syncthreads();
+    if (c.krem == 0) return;
+    if (to_lds) {
+      const uint32_t nc = nlds < lds_cap ? nlds : lds_cap;
+      refine_emit(d, c, LdsSrc{cbuf}, nc, lh, wsum, res, &ectr);
+    } else {
+      refine_emit(d, c, GlobalSrc{src}, n, lh, wsum, res, &ectr);
+    }
+  } else if (n <= lds_cap) {
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) cbuf[i] = src[i];
     __syncthreads();
     refine_emit(d, c, LdsSrc{cbuf}, n, lh, wsum, res, &ectr);
   } else {
-    __syncthreads();
     refine_emit(d, c, GlobalSrc{src}, n, lh, wsum, res, &ectr);
   }
 }
@@ -431,31 +710,137 @@ __device__ __forceinline__ void write_sorted(const TaskDesc& d, Src s, uint32_t 
 }
 
 // ---------------------------------------------------------------------------
-// Final sort of the staged results, one block per task.  LDS when m fits,
-// else in the task's (now free) spill region, which holds len >= m slots.
+// Register-blocked bitonic sort of P2 = BLOCK * E composites, E per thread
+// (thread t holds indices t*E .. t*E+E-1), ascending.  Strides below E run in
+// registers, strides inside a wave through lane shuffles, the two widest
+// (partner in another wave) through LDS scratch xs of P2 entries.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
+  const int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
+  const int hi = __shfl_xor((int)(uint32_t)(v >> 32), mask, 64);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+template <int E>
+__device__ __forceinline__ void reg_bitonic(uint64_t (&a)[E], uint64_t* xs) {
+  constexpr uint32_t P2 = (uint32_t)BLOCK * E;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t size = 2; size <= P2; size <<= 1) {
+    for (uint32_t j = size >> 1; j >= (uint32_t)E; j >>= 1) {
+      const uint32_t tj = j / E;  // partner thread = t ^ tj
+      if (tj < 64) {
+#pragma unroll
+        for (int l = 0; l < E; ++l) {
+          const uint64_t y = shfl_xor64(a[l], (int)tj);
+          const uint32_t i = t * E + l;
+          const bool keep_min = ((i & j) == 0) == ((i & size) == 0);
+          const uint64_t x = a[l];
+          a[l] = keep_min ? (x < y ? x : y) : (x < y ? y : x);
+        }
+      } else {
+#pragma unroll
+        for (int l = 0; l < E; ++l) xs[t * E + l] = a[l];
+        __syncthreads();
+#pragma unroll
+        for (int l = 0; l < E; ++l) {
+          const uint64_t y = xs[(t ^ tj) * E + l];
+          const uint32_t i = t * E + l;
+          const bool keep_min = ((i & j) == 0) == ((i & size) == 0);
+          const uint64_t x = a[l];
+          a[l] = keep_min ? (x < y ? x : y) : (x < y ? y : x);
+        }
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int jj = E / 2; jj > 0; jj >>= 1) {
+      if ((uint32_t)jj < size) {
+#pragma unroll
+        for (int l = 0; l < E; ++l) {
+          const int p = l ^ jj;
+          if (p > l) {
+            const bool asc = ((t * E + l) & size) == 0;
+            const uint64_t x = a[l], y = a[p];
+            const bool sw = asc ? (x > y) : (x < y);
+            a[l] = sw ? y : x;
+            a[p] = sw ? x : y;
+          }
+        }
+      }
+    }
+  }
+}
+
+// One output slot from a sorted composite (or padding when x is the +inf pad).
+__device__ __forceinline__ void write_slot(const TaskDesc& d, uint32_t i, uint64_t x, bool valid) {
+  float v = 0.0f;
+  int64_t id = -1;
+  const bool i64 = (d.flags & F_IDX64) != 0;
+  if (valid) {
+    const uint32_t am = (d.flags & F_ASCEND) ? 0u : 0xFFFFFFFFu;
+    uint32_t pos = (uint32_t)x;
+    if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
+    v = fkey_inv((uint32_t)(x >> 32) ^ am);
+    if (d.flags & F_WITHIDXIN)
+      id = i64 ? reinterpret_cast<const int64_t*>(d.idx_in)[pos]
+               : (int64_t)reinterpret_cast<const int32_t*>(d.idx_in)[pos];
+    else
+      id = (int64_t)pos + 1;
+  }
+  d.val_out[i] = v;
+  if (i64) reinterpret_cast<int64_t*>(d.idx_out)[i] = id;
+  else reinterpret_cast<int32_t*>(d.idx_out)[i] = (int32_t)id;
+}
+
+template <int E>
+__device__ __forceinline__ void sort_task_regs(const TaskDesc& d, uint32_t m, uint64_t* xs) {
+  const bool i64 = (d.flags & F_IDX64) != 0;
+  const uint32_t t = threadIdx.x;
+  uint64_t a[E];
+#pragma unroll
+  for (int l = 0; l < E; ++l) {
+    const uint32_t i = t * E + l;
+    if (i < m) {
+      const uint32_t pos = i64 ? (uint32_t)reinterpret_cast<const int64_t*>(d.idx_out)[i]
+                               : (uint32_t)reinterpret_cast<const int32_t*>(d.idx_out)[i];
+      a[l] = out_composite(d, d.val_out[i], pos);
+    } else {
+      a[l] = ~0ull;
+    }
+  }
+  reg_bitonic<E>(a, xs);  // its widest stages synchronise the block: every read above is done
+  const uint32_t k = (uint32_t)d.k;
+#pragma unroll
+  for (int l = 0; l < E; ++l) {
+    const uint32_t i = t * E + l;
+    if (i < k) write_slot(d, i, a[l], i < m);
+  }
+  for (uint32_t i = (uint32_t)BLOCK * E + t; i < k; i += BLOCK) write_slot(d, i, 0ull, false);
+}
+
+// ---------------------------------------------------------------------------
+// Final sort of the staged results, one block per task: registers + LDS up to
+// reg_cap results, else a bitonic in the task's (now free) spill region,
+// which holds len >= m slots.
 // ---------------------------------------------------------------------------
 struct GlobalPtr {
   uint64_t* __restrict__ p;
   __device__ uint64_t& operator[](uint32_t i) const { return p[i]; }
 };
 
+// E: results per thread, chosen by the host from the launch's largest m; the
+// dynamic LDS holds BLOCK * E entries.
+template <int E>
 __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tasks,
                                                 const TaskCtl* __restrict__ ctl,
-                                                uint64_t* __restrict__ cand, uint32_t lds_cap) {
+                                                uint64_t* __restrict__ cand) {
   extern __shared__ uint64_t sb[];
   const int t = blockIdx.x;
   const TaskDesc d = tasks[t];
   const uint32_t m = ctl[t].m;
   const bool i64 = (d.flags & F_IDX64) != 0;
-  if (m <= lds_cap) {
-    for (uint32_t i = threadIdx.x; i < m; i += BLOCK) {
-      const uint32_t pos = i64 ? (uint32_t)reinterpret_cast<const int64_t*>(d.idx_out)[i]
-                               : (uint32_t)reinterpret_cast<const int32_t*>(d.idx_out)[i];
-      sb[i] = out_composite(d, d.val_out[i], pos);
-    }
-    __syncthreads();
-    bitonic_sort_asc(sb, m);
-    write_sorted(d, (const uint64_t*)sb, m);
+  if (m <= (uint32_t)BLOCK * E) {
+    sort_task_regs<E>(d, m, sb);
   } else {
     uint64_t* __restrict__ g = cand + d.base;
     for (uint32_t i = threadIdx.x; i < m; i += BLOCK) {
@@ -534,12 +919,30 @@ __global__ __launch_bounds__(BLOCK) void k_diag_scan(const uint32_t* __restrict_
   }
 }
 
+// Plain streaming copy (16-byte loads and stores), the reference point for
+// HBM traffic counters: it reads and writes exactly `n16` * 16 bytes.
+__global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst,
+                                                     int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n16; i += (int64_t)gridDim.x * BLOCK)
+    dst[i] = __builtin_nontemporal_load(src + i);
+}
+
 // Explicit instantiations used by the host.
-template __global__ void k_hist<1>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, uint32_t*);
-template __global__ void k_hist<2>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, uint32_t*);
-template __global__ void k_hist<4>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, uint32_t*);
-template __global__ void k_compact<1>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, TaskCtl*, uint64_t*);
-template __global__ void k_compact<2>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, TaskCtl*, uint64_t*);
-template __global__ void k_compact<4>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, TaskCtl*, uint64_t*);
+#define RADIK_INST(U)                                                                                   \
+  template __global__ void k_hist<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, uint32_t*, \
+                                     TaskCtl*, uint64_t*);                                              \
+  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, int64_t,   \
+                                        TaskCtl*, uint64_t*);
+RADIK_INST(1)
+RADIK_INST(2)
+RADIK_INST(4)
+#undef RADIK_INST
+template __global__ void k_sort<1>(const TaskDesc*, const TaskCtl*, uint64_t*);
+template __global__ void k_sort<2>(const TaskDesc*, const TaskCtl*, uint64_t*);
+template __global__ void k_sort<4>(const TaskDesc*, const TaskCtl*, uint64_t*);
+template __global__ void k_sort<8>(const TaskDesc*, const TaskCtl*, uint64_t*);
+template __global__ void k_sort<16>(const TaskDesc*, const TaskCtl*, uint64_t*);
+template __global__ void k_sort<32>(const TaskDesc*, const TaskCtl*, uint64_t*);
+template __global__ void k_sort<64>(const TaskDesc*, const TaskCtl*, uint64_t*);
 
 }  // namespace radik

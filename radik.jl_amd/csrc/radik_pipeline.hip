@@ -54,7 +54,9 @@ int dev_facts(DevFacts** out) {
       const int want = (int)budget;
       bool ok = hipFuncSetAttribute((const void*)k_refine, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     want - 20 * 1024) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                hipFuncSetAttribute((const void*)k_sort<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_sort<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     want) == hipSuccess &&
                 hipFuncSetAttribute((const void*)k_small, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     want - 20 * 1024) == hipSuccess;
@@ -65,9 +67,15 @@ int dev_facts(DevFacts** out) {
     }
     f.lds_per_block = budget;
     const size_t stat = 20 * 1024;  // static LDS of k_refine / k_small (histogram + scratch)
-    f.refine_cap = (uint32_t)((budget - stat) / 8);
-    f.sort_cap = (uint32_t)(budget / 8);
-    f.small_cap = (uint32_t)((budget - stat) / 16);
+    f.refine_cap_max = (uint32_t)((budget - stat) / 8);
+    f.small_cap_max = (uint32_t)((budget - stat) / 16);
+    // Defaults keep k_refine and k_small at two or more blocks per CU.
+    f.refine_cap = std::min<uint32_t>(4096, f.refine_cap_max);
+    f.small_cap = std::min<uint32_t>(2048, f.small_cap_max);
+    // register-blocked sort: up to 64 per thread, LDS scratch of a power of two
+    uint32_t sc = BLOCK;
+    while (sc * 2 <= (uint32_t)(budget / 8) && sc * 2 <= (uint32_t)BLOCK * 64) sc *= 2;
+    f.sort_cap = sc;
     f.device = dev;
   }
   *out = &f;
@@ -80,6 +88,9 @@ int dev_facts(DevFacts** out) {
 Pipeline::~Pipeline() {
   if (h_large) (void)hipHostFree(h_large);
   if (h_bases) (void)hipHostFree(h_bases);
+  if (h_sbases) (void)hipHostFree(h_sbases);
+  if (d_sbases) (void)hipFree(d_sbases);
+  if (d_shist) (void)hipFree(d_shist);
   if (h_small) (void)hipHostFree(h_small);
   if (d_large) (void)hipFree(d_large);
   if (d_bases) (void)hipFree(d_bases);
@@ -106,14 +117,19 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
     size_t c = grow(cap_large, (size_t)nl);
     if (h_large) RADIK_HIP(hipHostFree(h_large));
     if (h_bases) RADIK_HIP(hipHostFree(h_bases));
+    if (h_sbases) RADIK_HIP(hipHostFree(h_sbases));
     if (d_large) RADIK_HIP(hipFree(d_large));
     if (d_bases) RADIK_HIP(hipFree(d_bases));
+    if (d_sbases) RADIK_HIP(hipFree(d_sbases));
     if (d_ctl) RADIK_HIP(hipFree(d_ctl));
-    h_large = nullptr; h_bases = nullptr; d_large = nullptr; d_bases = nullptr; d_ctl = nullptr;
+    h_large = nullptr; h_bases = nullptr; h_sbases = nullptr;
+    d_large = nullptr; d_bases = nullptr; d_sbases = nullptr; d_ctl = nullptr;
     if (hipHostMalloc((void**)&h_large, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_bases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_sbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&d_large, c * sizeof(TaskDesc)) != hipSuccess ||
         hipMalloc((void**)&d_bases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc((void**)&d_sbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_ctl, c * sizeof(TaskCtl)) != hipSuccess) {
       cap_large = 0;
       return set_error(6, "allocation of %zu task descriptors failed", c);
@@ -124,12 +140,16 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = grow(cap_hist, (size_t)nl);
     if (d_hist) RADIK_HIP(hipFree(d_hist));
+    if (d_shist) RADIK_HIP(hipFree(d_shist));
     d_hist = nullptr;
-    if (hipMalloc((void**)&d_hist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess) {
+    d_shist = nullptr;
+    if (hipMalloc((void**)&d_hist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&d_shist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess) {
       cap_hist = 0;
       return set_error(6, "allocation of %zu task histograms failed", c);
     }
     RADIK_HIP(hipMemset(d_hist, 0, c * HIST_BINS * sizeof(uint32_t)));
+    RADIK_HIP(hipMemset(d_shist, 0, c * HIST_BINS * sizeof(uint32_t)));
     cap_hist = c;
   }
   if ((size_t)ns > cap_small) {
@@ -159,23 +179,50 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
   return 0;
 }
 
-// Short tasks run whole in one block (k_small); small_cap 0 routes every
-// task through the streaming passes.
-static inline bool is_short(int64_t len, const DevFacts& f) {
-  return f.small_cap > 0 && len <= (int64_t)f.small_cap;
+// Sample groups (of SAMPLE_GROUP elements) read to predict a long task's
+// first-digit bound: about 1/64 of the task, 16..1024 groups.  Tasks shorter
+// than 2^16 elements, or whose k exceeds 1/64 of their length, are not
+// predicted and take the two-read schedule.
+static int64_t sample_groups(int64_t len, int64_t k) {
+  const int64_t m = k < len ? k : len;
+  if (len < (int64_t)1 << 16 || m <= 0 || m * 64 > len) return 0;
+  int64_t g = len / ((int64_t)SAMPLE_GROUP * 64);
+  if (g < 16) g = 16;
+  if (g > 1024) g = 1024;
+  return g;
 }
 
-int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, hipStream_t s) {
-  int64_t nl = 0, ns = 0, tot = 0;
+// Short tasks run whole in one block (k_small); small_cap 0 routes every
+// task through the streaming passes.
+static inline bool is_short(int64_t len, uint32_t small_cap) {
+  return small_cap > 0 && len <= (int64_t)small_cap;
+}
+
+// Long tasks are laid out predicted first, then the ones pass B always
+// re-reads, so pass B can cover just the latter's elements densely.
+int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
+                hipStream_t s) {
+  const uint32_t scap = eff_small_cap(f, o);
+  int64_t nl = 0, ns = 0, tot = 0, tot_pred = 0;
   for (int64_t t = 0; t < n; ++t) {
-    if (is_short(tasks[t].len, f)) ++ns;
-    else { ++nl; tot += tasks[t].len; }
+    if (is_short(tasks[t].len, scap)) {
+      ++ns;
+    } else {
+      ++nl;
+      tot += tasks[t].len;
+      if (o.predict && sample_groups(tasks[t].len, tasks[t].k)) tot_pred += tasks[t].len;
+    }
   }
   int rc = p.reserve(nl, ns, tot, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, base = 0;
+  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0;
+  // pass 0: predicted long tasks; pass 1: short tasks and the other long tasks
+  for (int pass = 0; pass < 2; ++pass)
   for (int64_t t = 0; t < n; ++t) {
     const TaskSpec& ts = tasks[t];
+    const bool shrt = is_short(ts.len, scap);
+    const bool pred = o.predict && !shrt && sample_groups(ts.len, ts.k) > 0;
+    if ((pass == 0) != pred) continue;
     TaskDesc d;
     d.in = ts.in;
     d.idx_in = ts.idx_in;
@@ -184,20 +231,28 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     d.len = ts.len;
     d.k = (int32_t)ts.k;
     d.flags = ts.flags;
-    if (is_short(ts.len, f)) {
+    if (shrt) {
       d.base = 0;
       p.h_small[is++] = d;
     } else {
       d.base = base;
       p.h_bases[il] = base;
+      p.h_sbases[il] = groups;
+      const int64_t m = ts.k < ts.len ? ts.k : ts.len;
+      if (m > max_m) max_m = m;
       p.h_large[il++] = d;
       base += ts.len;
+      groups += pred ? sample_groups(ts.len, ts.k) : 0;
     }
   }
   p.h_bases[il] = base;
+  p.h_sbases[il] = groups;
   p.n_large = nl;
   p.n_small = ns;
   p.total = tot;
+  p.total_pred = tot_pred;
+  p.total_groups = groups;
+  p.max_m = (uint32_t)max_m;
   return 0;
 }
 
@@ -205,6 +260,7 @@ int upload(Pipeline& p, hipStream_t s) {
   if (p.n_large) {
     RADIK_HIP(hipMemcpyAsync(p.d_large, p.h_large, p.n_large * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_bases, p.h_bases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   }
   if (p.n_small)
     RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
@@ -218,65 +274,110 @@ int upload(Pipeline& p, hipStream_t s) {
 template <int UNR>
 static void launch_stream_kernels_hist(const Pipeline& p, int64_t chunk, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_hist<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     p.total, chunk, p.d_hist);
+                     p.total, chunk, p.d_hist, p.d_ctl, p.d_cand);
 }
 
 template <int UNR>
-static void launch_stream_kernels_compact(const Pipeline& p, int64_t chunk, int grid, hipStream_t s) {
+static void launch_stream_kernels_compact(const Pipeline& p, int64_t g_begin, int64_t g_end, int64_t chunk,
+                                          int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_compact<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     p.total, chunk, p.d_ctl, p.d_cand);
+                     g_begin, g_end, chunk, p.d_ctl, p.d_cand);
+}
+
+// Equal chunks of `total` elements, about blocks_per_cu blocks per CU.
+static int64_t stream_chunk(int64_t total, const DevFacts& f, const LaunchOpts& o, int* grid) {
+  const int64_t want_blocks = (int64_t)f.cus * o.blocks_per_cu;
+  int64_t chunk = (total + want_blocks - 1) / want_blocks;
+  const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
+  if (chunk < 4 * gran) chunk = 4 * gran;
+  chunk = (chunk + gran - 1) / gran * gran;
+  *grid = (int)std::max<int64_t>(1, (total + chunk - 1) / chunk);
+  return chunk;
 }
 
 int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm) {
   const bool timed = tm && tm->on;
-  auto mark = [&](int i) -> int {
-    if (timed) RADIK_HIP(hipEventRecord(tm->ev[i], s));
+  int phase = 0;
+  auto mark = [&]() -> int {  // closes the current phase
+    if (timed) RADIK_HIP(hipEventRecord(tm->ev[phase], s));
+    ++phase;
     return 0;
   };
   int rc;
-  if ((rc = mark(0))) return rc;
+  if ((rc = mark())) return rc;  // start
   if (p.n_large) {
-    const int64_t want_blocks = (int64_t)f.cus * o.blocks_per_cu;
-    int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
-    const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
-    if (chunk < 4 * gran) chunk = 4 * gran;
-    chunk = (chunk + gran - 1) / gran * gran;
-    const int grid = (int)std::max<int64_t>(1, (p.total + chunk - 1) / chunk);
+    int grid = 0;
+    const int64_t chunk = stream_chunk(p.total, f, o, &grid);
+    if (p.total_groups > 0) {
+      const int sgrid = (int)((p.total_groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK);
+      hipLaunchKernelGGL(k_sample, dim3(sgrid), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
+                         p.total_groups, p.d_shist);
+      RADIK_HIP(hipGetLastError());
+    }
+    if ((rc = mark())) return rc;
+    hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
+                       p.d_ctl);
+    RADIK_HIP(hipGetLastError());
+    if ((rc = mark())) return rc;
     switch (o.unr) {
       case 1: launch_stream_kernels_hist<1>(p, chunk, grid, s); break;
       case 4: launch_stream_kernels_hist<4>(p, chunk, grid, s); break;
       default: launch_stream_kernels_hist<2>(p, chunk, grid, s); break;
     }
     RADIK_HIP(hipGetLastError());
-    if ((rc = mark(1))) return rc;
+    if ((rc = mark())) return rc;
     hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl);
     RADIK_HIP(hipGetLastError());
-    if ((rc = mark(2))) return rc;
-    switch (o.unr) {
-      case 1: launch_stream_kernels_compact<1>(p, chunk, grid, s); break;
-      case 4: launch_stream_kernels_compact<4>(p, chunk, grid, s); break;
-      default: launch_stream_kernels_compact<2>(p, chunk, grid, s); break;
+    if ((rc = mark())) return rc;
+    // Pass B: the tasks without a prediction (all of them re-read), then the
+    // predicted ones (blocks skip tasks whose spill covered the bin).
+    for (int part = 0; part < 2; ++part) {
+      const int64_t gb = part == 0 ? p.total_pred : 0;
+      const int64_t ge = part == 0 ? p.total : p.total_pred;
+      if (ge <= gb) continue;
+      int g2 = 0;
+      const int64_t c2 = stream_chunk(ge - gb, f, o, &g2);
+      switch (o.unr) {
+        case 1: launch_stream_kernels_compact<1>(p, gb, ge, c2, g2, s); break;
+        case 4: launch_stream_kernels_compact<4>(p, gb, ge, c2, g2, s); break;
+        default: launch_stream_kernels_compact<2>(p, gb, ge, c2, g2, s); break;
+      }
+      RADIK_HIP(hipGetLastError());
+    }
+    if ((rc = mark())) return rc;
+    const uint32_t rcap = eff_refine_cap(f, o);
+    hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)rcap * 8, s, p.d_large,
+                       p.d_ctl, p.d_cand, rcap);
+    RADIK_HIP(hipGetLastError());
+    if ((rc = mark())) return rc;
+    {
+      // results per thread of the register sort: the launch's largest m
+      uint32_t e = 1;
+      while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;
+      const size_t lds = (size_t)BLOCK * e * 8;
+      const dim3 g((unsigned)p.n_large), b(BLOCK);
+      switch (e) {
+        case 1: hipLaunchKernelGGL(k_sort<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 2: hipLaunchKernelGGL(k_sort<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 4: hipLaunchKernelGGL(k_sort<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 8: hipLaunchKernelGGL(k_sort<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 16: hipLaunchKernelGGL(k_sort<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        case 32: hipLaunchKernelGGL(k_sort<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        default: hipLaunchKernelGGL(k_sort<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+      }
     }
     RADIK_HIP(hipGetLastError());
-    if ((rc = mark(3))) return rc;
-    hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)f.refine_cap * 8, s,
-                       p.d_large, p.d_ctl, p.d_cand, f.refine_cap);
-    RADIK_HIP(hipGetLastError());
-    if ((rc = mark(4))) return rc;
-    hipLaunchKernelGGL(k_sort, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)f.sort_cap * 8, s, p.d_large,
-                       p.d_ctl, p.d_cand, f.sort_cap);
-    RADIK_HIP(hipGetLastError());
   } else {
-    for (int i = 1; i <= 4; ++i)
-      if ((rc = mark(i))) return rc;
+    for (int i = 0; i < 6; ++i)
+      if ((rc = mark())) return rc;
   }
-  if ((rc = mark(5))) return rc;
+  if ((rc = mark())) return rc;  // end of sort
   if (p.n_small) {
-    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)f.small_cap * 16, s,
-                       p.d_small, f.small_cap);
+    const uint32_t scap = eff_small_cap(f, o);
+    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, s, p.d_small, scap);
     RADIK_HIP(hipGetLastError());
   }
-  if ((rc = mark(6))) return rc;
+  if ((rc = mark())) return rc;  // end of small
   return 0;
 }
 
@@ -289,14 +390,29 @@ int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStre
   return 0;
 }
 
+int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
+  const int64_t n16 = bytes / 16;
+  if (n16 <= 0) return 0;
+  int dev = 0, cus = 0;
+  RADIK_HIP(hipGetDevice(&dev));
+  RADIK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t want = (int64_t)(cus > 0 ? cus : 256) * 16;
+  const int grid = (int)std::min<int64_t>(want, (n16 + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_diag_copy, dim3(grid), dim3(BLOCK), 0, s, (const f32x4*)src, (f32x4*)dst, n16);
+  RADIK_HIP(hipGetLastError());
+  return 0;
+}
+
 int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s) {
   if (!p.n_large) return 0;
-  const int64_t want_blocks = (int64_t)f.cus * 8;
-  int64_t chunk = (p.total + want_blocks - 1) / want_blocks;
-  const int64_t gran = (int64_t)BLOCK * 8;
-  if (chunk < 4 * gran) chunk = 4 * gran;
-  chunk = (chunk + gran - 1) / gran * gran;
-  const int grid = (int)std::max<int64_t>(1, (p.total + chunk - 1) / chunk);
+  LaunchOpts o;
+  int grid = 0;
+  const int64_t chunk = stream_chunk(p.total, f, o, &grid);
+  // Control blocks without a prediction: pass A only counts.
+  // staged without prediction: k_predict only initialises the control blocks
+  hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
+                     p.d_ctl);
+  RADIK_HIP(hipGetLastError());
   launch_stream_kernels_hist<2>(p, chunk, grid, s);
   RADIK_HIP(hipGetLastError());
   RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),

@@ -49,7 +49,8 @@ struct Slot {
   Pipeline pipe;
   std::vector<TaskSpec> specs;
   std::vector<int64_t> tickets;
-  bool busy = false;
+  bool busy = false;      // taken by the dispatcher
+  bool launched = false;  // its batch and completion event are enqueued
 };
 
 }  // namespace
@@ -110,6 +111,7 @@ static void dispatch_loop(radik_server* s) {
       s->pending.pop_front();
     }
     slot->busy = true;
+    slot->launched = false;
     s->inflight++;
     lk.unlock();
     slot->specs.clear();
@@ -132,7 +134,7 @@ static void dispatch_loop(radik_server* s) {
       slot->tickets.push_back(r.ticket);
     }
     LaunchOpts o;
-    int rc = stage_tasks(slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, slot->stream);
+    int rc = stage_tasks(slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, o, slot->stream);
     if (!rc) rc = upload(slot->pipe, slot->stream);
     if (!rc) rc = enqueue(slot->pipe, *f, o, slot->stream, nullptr);
     if (!rc && hipEventRecord(slot->done, slot->stream) != hipSuccess) rc = RADIK_EHIP;
@@ -141,13 +143,13 @@ static void dispatch_loop(radik_server* s) {
     int64_t prev = s->max_coalesced.load();
     while ((int64_t)batch.size() > prev && !s->max_coalesced.compare_exchange_weak(prev, (int64_t)batch.size())) {
     }
+    if (rc) (void)hipEventRecord(slot->done, slot->stream);  // released once the stream drains
     lk.lock();
     if (rc) {
       s->error = rc;
       s->error_msg = "batch launch failed";
-      // The completer still releases the slot once the stream drains.
-      (void)hipEventRecord(slot->done, slot->stream);
     }
+    slot->launched = true;
     s->cv_work.notify_all();
   }
 }
@@ -162,7 +164,7 @@ static void complete_loop(radik_server* s) {
       for (auto& sl : s->slots) {
         if (!sl->busy) continue;
         any = true;
-        if (hipEventQuery(sl->done) != hipSuccess) continue;
+        if (!sl->launched || hipEventQuery(sl->done) != hipSuccess) continue;
         const auto now = Clock::now();
         for (int64_t t : sl->tickets) {
           auto it = s->results.find(t);
@@ -170,6 +172,7 @@ static void complete_loop(radik_server* s) {
           it->second.done = true;
         }
         sl->busy = false;
+        sl->launched = false;
         s->inflight--;
         // latency: stamped against the submit time kept in the request map
         for (int64_t t : sl->tickets) {

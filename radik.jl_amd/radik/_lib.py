@@ -54,6 +54,7 @@ SIGNATURES = {
     "radik_plan_create": (_INT, [ctypes.POINTER(_P), _I64P, _I64P, _I64, _U32, _INT, _P]),
     "radik_plan_execute": (_INT, [_P, _P, _P, _P, _P, _P]),
     "radik_plan_set_graph": (_INT, [_P, _INT]),
+    "radik_plan_set_options": (_INT, [_P, _INT, _INT, _INT, _INT, _INT]),
     "radik_plan_set_timing": (_INT, [_P, _INT]),
     "radik_plan_timing": (_INT, [_P, ctypes.POINTER(ctypes.c_double), _I64P]),
     "radik_plan_destroy": (_INT, [_P]),
@@ -66,6 +67,7 @@ SIGNATURES = {
     "radik_server_destroy": (_INT, [_P]),
     "radik_diag_pass1": (_INT, [_P, _I64P, _I64, _I64, _U32, _P, _P, _P]),
     "radik_diag_scan": (_INT, [_P, _P, _P, _I64, _P]),
+    "radik_diag_copy": (_INT, [_P, _P, _I64, _P]),
     "radik_device_count": (_INT, [ctypes.POINTER(_INT)]),
     "radik_set_device": (_INT, [_INT]),
     "radik_device_info": (_INT, [ctypes.POINTER(_INT), _I64P, _I64P, ctypes.c_char_p, _INT]),

@@ -15,7 +15,7 @@ from .device import DeviceArray
 
 class Plan:
     def __init__(self, task_lens, k, largest=True, rev=False, withscale=False, indices_dtype=None,
-                 idx_dtype=np.int32, use_graph=True, stream=None):
+                 idx_dtype=np.int32, use_graph=True, stream=None, tuned=True):
         lens = [int(x) for x in task_lens]
         ks = [int(k)] * len(lens) if np.isscalar(k) else [int(x) for x in k]
         if len(ks) != len(lens):
@@ -35,6 +35,10 @@ class Plan:
                                                  None if stream is None else stream.handle))
         self.handle = h.value
         _lib.check(_lib.load().radik_plan_set_graph(self.handle, 1 if use_graph else 0))
+        self.options = None
+        if tuned:
+            from . import autotune
+            self.options = autotune.apply(self)
 
     @property
     def out_size(self):
@@ -45,14 +49,19 @@ class Plan:
                                                   None if idx_in is None else idx_in.ptr,
                                                   None if stream is None else stream.handle))
 
+    def set_options(self, vload_bytes=0, blocks_per_cu=0, small_cap=0, refine_cap=0, predict=True):
+        """Tuning knobs of the launch sequence; results do not depend on them."""
+        _lib.check(_lib.load().radik_plan_set_options(self.handle, int(vload_bytes), int(blocks_per_cu),
+                                                      int(small_cap), int(refine_cap), 1 if predict else 0))
+
     def set_timing(self, on):
         _lib.check(_lib.load().radik_plan_set_timing(self.handle, 1 if on else 0))
 
     def timing(self):
-        ms = (ctypes.c_double * 6)()
+        ms = (ctypes.c_double * 8)()
         n = ctypes.c_int64()
         _lib.check(_lib.load().radik_plan_timing(self.handle, ms, ctypes.byref(n)))
-        names = ("hist", "select", "compact", "refine", "sort", "small")
+        names = ("sample", "predict", "hist", "select", "compact", "refine", "sort", "small")
         return {nm: ms[i] for i, nm in enumerate(names)}, n.value
 
     def __del__(self):

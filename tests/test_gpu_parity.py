@@ -256,3 +256,35 @@ def test_large_single_task_properties():
             say("large", k, flags)
             got = run_ws(x, [n], k, flags, idx=perm)
             assert_same(got, oracle_topk(x, [n], k, flags, idx=perm), f"2^24 k{k} f{flags}")
+
+
+def test_predicted_pass_covers_ties_and_specials():
+    # long tasks (>= 2^17, k <= len/64) take the sampled single-read path
+    rng = np.random.default_rng(31)
+    n = 1 << 18
+    x = rng.standard_normal(4 * n).astype(np.float32)
+    x[n:2 * n][rng.random(n) < 0.3] = np.float32(3.5)          # heavy tie near the threshold
+    x[2 * n:3 * n] = np.round(x[2 * n:3 * n] * 2) / 2            # few levels
+    sp = rng.integers(3 * n, 4 * n, 500)
+    x[sp] = np.array([0.0, -0.0, np.inf, -np.inf, 1.4e-45], np.float32)[rng.integers(0, 5, 500)]
+    lens = [n, n, n, n]
+    for k in (1, 256, 4096):
+        for flags in (O.LARGEST | O.ASCEND, 0, O.LARGEST | O.WITHSCALE):
+            say("predicted", k, flags)
+            assert_same(run_ws(x, lens, k, flags), oracle_topk(x, lens, k, flags), f"pred k{k} f{flags}")
+
+
+def test_prediction_miss_forces_reread():
+    # The sampled positions (group j of G starts at j * len / G) hold large
+    # values that are absent elsewhere, so the sample overstates how many
+    # large keys the task has and the predicted bound is too tight; the
+    # fallback re-read must still give the exact result.
+    n = 1 << 20
+    k = n // 64
+    G = 64
+    x = np.random.default_rng(3).standard_normal(n).astype(np.float32)
+    for j in range(G):
+        st = j * n // G
+        x[st:st + 256] = 100.0 + np.arange(j * 256, j * 256 + 256, dtype=np.float32)
+    say("prediction miss")
+    assert_same(run_ws(x, [n], k, O.LARGEST | O.ASCEND), oracle_topk(x, [n], k, O.LARGEST | O.ASCEND), "miss")
