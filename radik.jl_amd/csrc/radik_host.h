@@ -57,11 +57,15 @@ struct Pipeline {
   TaskDesc* h_large = nullptr;
   int64_t* h_bases = nullptr;   // element prefix sums of the long tasks
   int64_t* h_sbases = nullptr;  // sample-group prefix sums of the long tasks
+  int64_t* h_cbases = nullptr;  // sorted-run prefix sums (multi-block sort)
+  int64_t* h_mbases = nullptr;  // result-count prefix sums (multi-block sort)
   TaskDesc* h_small = nullptr;
   // device
   TaskDesc* d_large = nullptr;
   int64_t* d_bases = nullptr;
   int64_t* d_sbases = nullptr;
+  int64_t* d_cbases = nullptr;
+  int64_t* d_mbases = nullptr;
   TaskDesc* d_small = nullptr;
   TaskCtl* d_ctl = nullptr;
   uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
@@ -72,6 +76,7 @@ struct Pipeline {
   int64_t n_large = 0, n_small = 0, total = 0, total_groups = 0;
   int64_t total_pred = 0;  // elements of the predicted tasks, which come first
   uint32_t max_m = 0;  // largest min(k, len) over the long tasks
+  int64_t total_runs = 0, total_m = 0;  // multi-block sort sizes
   hipEvent_t upload_done = nullptr;  // pinned buffers reusable after this
   ~Pipeline();
   int reserve(int64_t n_large, int64_t n_small, int64_t total_large, hipStream_t s);

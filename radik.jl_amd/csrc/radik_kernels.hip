@@ -492,6 +492,26 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
   }
 }
 
+// Loads of a candidate loop are batched LOADS_IN_FLIGHT per thread so one
+// block keeps enough requests in flight when the candidates sit in global
+// memory.
+constexpr int LOADS_IN_FLIGHT = 8;
+
+template <typename Src, typename F>
+__device__ __forceinline__ void for_each_entry(Src src, uint32_t n, F&& f) {
+  for (uint32_t b = threadIdx.x; b < n; b += BLOCK * LOADS_IN_FLIGHT) {
+    uint64_t xs[LOADS_IN_FLIGHT];
+#pragma unroll
+    for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+      const uint32_t i = b + (uint32_t)u * BLOCK;
+      xs[u] = (i < n) ? src[i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < LOADS_IN_FLIGHT; ++u)
+      if (b + (uint32_t)u * BLOCK < n) f(xs[u]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Digit selection over composites (shared by k_refine and k_small).
 // Digits below the first: rank bits [19:8], [7:0], then position bits
@@ -500,11 +520,14 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
 // to take, finds the shortest prefix at which the bin holding the need-th
 // smallest is taken whole.  Returns the shift s: the chosen composites are
 // exactly those with (x >> s) <= P (P updated), among those matching the
-// initial prefix.
+// initial prefix.  `vary` has a bit set where the prefix-matching composites
+// differ and `same` holds their common bits; a digit without varying bits has
+// a single bin and is fixed without a pass (exact ties skip the rank digits).
 // ---------------------------------------------------------------------------
 template <typename Src>
 __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint64_t& P, uint32_t need,
-                                             uint32_t* lh, uint32_t* wsum, uint32_t* res) {
+                                             uint64_t vary, uint64_t same, uint32_t* lh, uint32_t* wsum,
+                                             uint32_t* res) {
   const int shifts[6] = {52, 40, 32, 20, 8, 0};
   const int widths[6] = {12, 12, 8, 12, 12, 8};
   int s_final = top;
@@ -513,14 +536,19 @@ __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint6
     const int w = widths[q];
     if (s + w > top) continue;  // digit already fixed
     const uint32_t nb = 1u << w;
+    const int hs = s + w;
+    if (((vary >> s) & (nb - 1)) == 0) {  // one bin: every match carries this digit
+      P = (hs >= 64 ? 0ull : (P << w)) | ((same >> s) & (nb - 1));
+      s_final = s;
+      continue;
+    }
     for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) lh[i] = 0;
     __syncthreads();
-    const int hs = s + w;
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-      const uint64_t x = src[i];
-      const bool match = (hs >= 64) ? true : ((x >> hs) == P);
+    const uint64_t Pc = P;
+    for_each_entry(src, n, [&](uint64_t x) {
+      const bool match = (hs >= 64) ? true : ((x >> hs) == Pc);
       if (match) hist_add(lh, (uint32_t)(x >> s) & (nb - 1));
-    }
+    });
     __syncthreads();
     if (w == 12) {
       uint32_t c[16];
@@ -540,6 +568,14 @@ __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint6
     if (need == cnt) break;
   }
   return s_final;
+}
+
+// Block-wide AND and OR of the composites each thread saw (a, o): vb[0] and
+// vb[1] in LDS, initialised by the caller to ~0 and 0 before a barrier.
+__device__ __forceinline__ void block_and_or(uint64_t a, uint64_t o, unsigned long long* vb) {
+  atomicAnd(&vb[0], (unsigned long long)a);
+  atomicOr(&vb[1], (unsigned long long)o);
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -567,13 +603,15 @@ __device__ __forceinline__ float spilled_value(const TaskDesc& d, uint64_t x, ui
 
 template <typename Src>
 __device__ __forceinline__ void refine_emit(const TaskDesc& d, const TaskCtl& c, Src src, uint32_t n,
-                                            uint32_t* lh, uint32_t* wsum, uint32_t* res, uint32_t* ectr) {
+                                            const unsigned long long* vb, uint32_t* lh, uint32_t* wsum,
+                                            uint32_t* res, uint32_t* ectr) {
   uint64_t P = c.b_cand;
-  const int s = select_digits(src, n, 52, P, c.krem, lh, wsum, res);
+  const uint64_t vary = (uint64_t)(vb[0] ^ vb[1]);
+  const int s = select_digits(src, n, 52, P, c.krem, vary, (uint64_t)vb[0], lh, wsum, res);
   const uint32_t lim = c.m - c.nwin;
-  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-    const uint64_t x = src[i];
-    if ((x >> 52) == c.b_cand && (x >> s) <= P) {
+  const uint64_t bc = c.b_cand;
+  for_each_entry(src, n, [&](uint64_t x) {
+    if ((x >> 52) == bc && (x >> s) <= P) {
       const uint32_t slot = atomicAdd(ectr, 1u);
       if (slot < lim) {
         uint32_t pos;
@@ -581,7 +619,7 @@ __device__ __forceinline__ void refine_emit(const TaskDesc& d, const TaskCtl& c,
         put_result(d, c.nwin + slot, v, pos);
       }
     }
-  }
+  });
 }
 
 __global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ tasks,
@@ -593,6 +631,7 @@ __global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ t
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
   __shared__ uint32_t ectr, wctr, nlds;
+  __shared__ unsigned long long vb[2];
   const int t = blockIdx.x;
   const TaskCtl c = ctl[t];
   if (c.m == 0 || (!c.spill_all && c.krem == 0)) return;
@@ -603,14 +642,17 @@ __global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ t
     ectr = 0;
     wctr = 0;
     nlds = 0;
+    vb[0] = ~0ull;
+    vb[1] = 0ull;
   }
   __syncthreads();
   const uint64_t* __restrict__ src = cand + d.base;
+  const uint64_t bc = c.b_cand;
+  uint64_t a = ~0ull, o = 0ull;  // AND / OR of this thread's candidates
   if (c.spill_all) {
     // Split the spill: winners to the output slots, candidates to LDS.
     const bool to_lds = c.krem > 0 && c.ncand <= lds_cap;
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-      const uint64_t x = src[i];
+    for_each_entry(GlobalSrc{src}, n, [&](uint64_t x) {
       const uint32_t dg = (uint32_t)(x >> 52);
       if (dg < c.b_win) {
         const uint32_t slot = atomicAdd(&wctr, 1u);
@@ -619,25 +661,39 @@ __global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ t
           const float v = spilled_value(d, x, pos);
           put_result(d, slot, v, pos);
         }
-      } else if (to_lds && dg == c.b_cand) {
-        const uint32_t j = atomicAdd(&nlds, 1u);
-        if (j < lds_cap) cbuf[j] = x;
+      } else if (dg == bc) {
+        a &= x;
+        o |= x;
+        if (to_lds) {
+          const uint32_t j = atomicAdd(&nlds, 1u);
+          if (j < lds_cap) cbuf[j] = x;
+        }
       }
-    }
+    });
     __syncthreads();
     if (c.krem == 0) return;
+    block_and_or(a, o, vb);
     if (to_lds) {
       const uint32_t nc = nlds < lds_cap ? nlds : lds_cap;
-      refine_emit(d, c, LdsSrc{cbuf}, nc, lh, wsum, res, &ectr);
+      refine_emit(d, c, LdsSrc{cbuf}, nc, vb, lh, wsum, res, &ectr);
     } else {
-      refine_emit(d, c, GlobalSrc{src}, n, lh, wsum, res, &ectr);
+      refine_emit(d, c, GlobalSrc{src}, n, vb, lh, wsum, res, &ectr);
     }
   } else if (n <= lds_cap) {
+    for_each_entry(GlobalSrc{src}, n, [&](uint64_t x) {
+      a &= x;
+      o |= x;
+    });
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) cbuf[i] = src[i];
-    __syncthreads();
-    refine_emit(d, c, LdsSrc{cbuf}, n, lh, wsum, res, &ectr);
+    block_and_or(a, o, vb);
+    refine_emit(d, c, LdsSrc{cbuf}, n, vb, lh, wsum, res, &ectr);
   } else {
-    refine_emit(d, c, GlobalSrc{src}, n, lh, wsum, res, &ectr);
+    for_each_entry(GlobalSrc{src}, n, [&](uint64_t x) {
+      a &= x;
+      o |= x;
+    });
+    block_and_or(a, o, vb);
+    refine_emit(d, c, GlobalSrc{src}, n, vb, lh, wsum, res, &ectr);
   }
 }
 
@@ -858,6 +914,91 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
 }
 
 // ---------------------------------------------------------------------------
+// Sort of large result sets (m > 4096) over several blocks: k_sort_chunks
+// sorts runs of SORT_CHUNK results per block (register bitonic) into the
+// task's spill region; k_sort_rank then places every result at its final
+// slot = its rank in its own run + the number of smaller results in each
+// other run (binary searches, keys are distinct) and writes the outputs.
+// cbases: prefix sums of the runs per task; mbases: of the results per task.
+// ---------------------------------------------------------------------------
+constexpr uint32_t SORT_CHUNK = BLOCK * 8;
+constexpr uint32_t SORT_MAX_RUNS = 64;
+
+__global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restrict__ tasks,
+                                                       const TaskCtl* __restrict__ ctl,
+                                                       const int64_t* __restrict__ cbases, int ntasks,
+                                                       uint64_t* __restrict__ cand) {
+  __shared__ uint64_t xs[SORT_CHUNK];
+  const int t = find_task(cbases, ntasks, blockIdx.x);
+  const TaskDesc d = tasks[t];
+  const uint32_t m = ctl[t].m;
+  const uint32_t j0 = (uint32_t)(blockIdx.x - cbases[t]) * SORT_CHUNK;
+  const bool i64 = (d.flags & F_IDX64) != 0;
+  uint64_t a[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    const uint32_t i = j0 + threadIdx.x * 8 + l;
+    if (i < m) {
+      const uint32_t pos = i64 ? (uint32_t)reinterpret_cast<const int64_t*>(d.idx_out)[i]
+                               : (uint32_t)reinterpret_cast<const int32_t*>(d.idx_out)[i];
+      a[l] = out_composite(d, d.val_out[i], pos);
+    } else {
+      a[l] = ~0ull;
+    }
+  }
+  reg_bitonic<8>(a, xs);
+  uint64_t* __restrict__ g = cand + d.base;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    const uint32_t i = j0 + threadIdx.x * 8 + l;
+    if (i < m) g[i] = a[l];
+  }
+  if (j0 == 0)  // padding slots past m
+    for (uint32_t i = m + threadIdx.x; i < (uint32_t)d.k; i += BLOCK) write_slot(d, i, 0ull, false);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict__ tasks,
+                                                     const TaskCtl* __restrict__ ctl,
+                                                     const int64_t* __restrict__ mbases, int ntasks,
+                                                     int64_t total, const uint64_t* __restrict__ cand) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= total) return;
+  const int t = find_task(mbases, ntasks, e);
+  const TaskDesc d = tasks[t];
+  const uint32_t m = ctl[t].m;
+  const uint32_t j = (uint32_t)(e - mbases[t]);
+  if (j >= m) return;
+  const uint64_t* __restrict__ g = cand + d.base;
+  const uint64_t x = g[j];
+  const uint32_t own = j / SORT_CHUNK;
+  const uint32_t runs = (m + SORT_CHUNK - 1) / SORT_CHUNK;
+  uint32_t pos = j - own * SORT_CHUNK;
+  // lower bounds of x in the other runs, eight searches in lockstep
+  for (uint32_t r0 = 0; r0 < runs; r0 += 8) {
+    uint32_t lo[8], hi[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t r = r0 + u;
+      lo[u] = 0;
+      hi[u] = (r < runs && r != own) ? min(SORT_CHUNK, m - r * SORT_CHUNK) : 0u;
+    }
+    for (int step = 0; step < 12; ++step) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (lo[u] < hi[u]) {
+          const uint32_t mid = (lo[u] + hi[u]) >> 1;
+          if (g[(r0 + u) * SORT_CHUNK + mid] < x) lo[u] = mid + 1;
+          else hi[u] = mid;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pos += lo[u];
+  }
+  if (pos < m) write_slot(d, pos, x, true);
+}
+
+// ---------------------------------------------------------------------------
 // Short tasks: one block does the whole top-k in LDS.  Dynamic LDS holds
 // 2 * cap composites (the task, then the chosen ones).
 // ---------------------------------------------------------------------------
@@ -867,6 +1008,7 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
   __shared__ uint32_t ectr;
+  __shared__ unsigned long long vb[2];
   const TaskDesc d = tasks[blockIdx.x];
   const uint32_t n = (uint32_t)d.len;
   const uint32_t m = d.k < (int32_t)n ? (uint32_t)d.k : n;
@@ -877,11 +1019,23 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
   const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
     all[i] = ((uint64_t)rank_key(d.in[i], sc, xm) << 32) | i;
-  if (threadIdx.x == 0) ectr = 0;
+  if (threadIdx.x == 0) {
+    ectr = 0;
+    vb[0] = ~0ull;
+    vb[1] = 0ull;
+  }
   __syncthreads();
   int s = 64;
   uint64_t P = 0;
-  if (m > 0 && m < n) s = select_digits(LdsSrc{all}, n, 64, P, m, lh, wsum, res);
+  if (m > 0 && m < n) {
+    uint64_t a = ~0ull, o = 0ull;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+      a &= all[i];
+      o |= all[i];
+    }
+    block_and_or(a, o, vb);
+    s = select_digits(LdsSrc{all}, n, 64, P, m, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh, wsum, res);
+  }
   const bool scale = (d.flags & F_WITHSCALE) != 0;
   if (m > 0) {
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {

@@ -90,6 +90,10 @@ Pipeline::~Pipeline() {
   if (h_bases) (void)hipHostFree(h_bases);
   if (h_sbases) (void)hipHostFree(h_sbases);
   if (d_sbases) (void)hipFree(d_sbases);
+  if (h_cbases) (void)hipHostFree(h_cbases);
+  if (d_cbases) (void)hipFree(d_cbases);
+  if (h_mbases) (void)hipHostFree(h_mbases);
+  if (d_mbases) (void)hipFree(d_mbases);
   if (d_shist) (void)hipFree(d_shist);
   if (h_small) (void)hipHostFree(h_small);
   if (d_large) (void)hipFree(d_large);
@@ -118,15 +122,24 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
     if (h_large) RADIK_HIP(hipHostFree(h_large));
     if (h_bases) RADIK_HIP(hipHostFree(h_bases));
     if (h_sbases) RADIK_HIP(hipHostFree(h_sbases));
+    if (h_cbases) RADIK_HIP(hipHostFree(h_cbases));
+    if (h_mbases) RADIK_HIP(hipHostFree(h_mbases));
     if (d_large) RADIK_HIP(hipFree(d_large));
     if (d_bases) RADIK_HIP(hipFree(d_bases));
     if (d_sbases) RADIK_HIP(hipFree(d_sbases));
+    if (d_cbases) RADIK_HIP(hipFree(d_cbases));
+    if (d_mbases) RADIK_HIP(hipFree(d_mbases));
     if (d_ctl) RADIK_HIP(hipFree(d_ctl));
-    h_large = nullptr; h_bases = nullptr; h_sbases = nullptr;
-    d_large = nullptr; d_bases = nullptr; d_sbases = nullptr; d_ctl = nullptr;
+    h_large = nullptr; h_bases = nullptr; h_sbases = nullptr; h_cbases = nullptr; h_mbases = nullptr;
+    d_large = nullptr; d_bases = nullptr; d_sbases = nullptr; d_cbases = nullptr; d_mbases = nullptr;
+    d_ctl = nullptr;
     if (hipHostMalloc((void**)&h_large, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_bases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_sbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_cbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_mbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_cbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc((void**)&d_mbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_large, c * sizeof(TaskDesc)) != hipSuccess ||
         hipMalloc((void**)&d_bases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_sbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
@@ -215,7 +228,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   }
   int rc = p.reserve(nl, ns, tot, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0;
+  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0, runs = 0, msum = 0;
   // pass 0: predicted long tasks; pass 1: short tasks and the other long tasks
   for (int pass = 0; pass < 2; ++pass)
   for (int64_t t = 0; t < n; ++t) {
@@ -240,6 +253,10 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
       p.h_sbases[il] = groups;
       const int64_t m = ts.k < ts.len ? ts.k : ts.len;
       if (m > max_m) max_m = m;
+      p.h_cbases[il] = runs;
+      p.h_mbases[il] = msum;
+      runs += std::max<int64_t>(1, (m + SORT_CHUNK - 1) / SORT_CHUNK);  // run 0 also pads
+      msum += m;
       p.h_large[il++] = d;
       base += ts.len;
       groups += pred ? sample_groups(ts.len, ts.k) : 0;
@@ -247,6 +264,10 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   }
   p.h_bases[il] = base;
   p.h_sbases[il] = groups;
+  p.h_cbases[il] = runs;
+  p.h_mbases[il] = msum;
+  p.total_runs = runs;
+  p.total_m = msum;
   p.n_large = nl;
   p.n_small = ns;
   p.total = tot;
@@ -261,6 +282,8 @@ int upload(Pipeline& p, hipStream_t s) {
     RADIK_HIP(hipMemcpyAsync(p.d_large, p.h_large, p.n_large * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_bases, p.h_bases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    RADIK_HIP(hipMemcpyAsync(p.d_cbases, p.h_cbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    RADIK_HIP(hipMemcpyAsync(p.d_mbases, p.h_mbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   }
   if (p.n_small)
     RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
@@ -350,7 +373,15 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
                        p.d_ctl, p.d_cand, rcap);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    {
+    if (p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS) {
+      // several blocks per task: sorted runs, then ranks across runs
+      hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)p.total_runs), dim3(BLOCK), 0, s, p.d_large, p.d_ctl,
+                         p.d_cbases, (int)p.n_large, p.d_cand);
+      RADIK_HIP(hipGetLastError());
+      const unsigned rg = (unsigned)((p.total_m + BLOCK - 1) / BLOCK);
+      hipLaunchKernelGGL(k_sort_rank, dim3(rg), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_mbases,
+                         (int)p.n_large, p.total_m, (const uint64_t*)p.d_cand);
+    } else {
       // results per thread of the register sort: the launch's largest m
       uint32_t e = 1;
       while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;

@@ -288,3 +288,34 @@ def test_prediction_miss_forces_reread():
         x[st:st + 256] = 100.0 + np.arange(j * 256, j * 256 + 256, dtype=np.float32)
     say("prediction miss")
     assert_same(run_ws(x, [n], k, O.LARGEST | O.ASCEND), oracle_topk(x, [n], k, O.LARGEST | O.ASCEND), "miss")
+
+
+def test_large_k_multi_block_sort():
+    # k above 4096 sorts in several blocks (runs + ranks); ties and specials
+    rng = np.random.default_rng(16384)
+    n = 1 << 20
+    x = rng.standard_normal(n).astype(np.float32)
+    x[rng.integers(0, n, 20000)] = np.float32(2.5)  # a heavy tie inside the top-k
+    x[rng.integers(0, n, 300)] = np.array([np.inf, -0.0, 0.0], np.float32)[rng.integers(0, 3, 300)]
+    for k in (4097, 16384, 70000):
+        for flags in (O.LARGEST | O.ASCEND, O.LARGEST, 0):
+            say("large k", k, flags)
+            assert_same(run_ws(x, [n], k, flags), oracle_topk(x, [n], k, flags), f"k{k} f{flags}")
+    # one launch mixing result sizes below and above 4096 (per-task k)
+    lens = [n, 300000, 70000, 9000]
+    ks = [20000, 100, 5000, 9000]
+    xs = rng.standard_normal(sum(lens)).astype(np.float32)
+    p = radik.Plan(lens, ks, largest=True, rev=False)
+    d = DeviceArray.from_host(xs)
+    vo = DeviceArray.zeros(sum(ks), np.float32)
+    io = DeviceArray.zeros(sum(ks), np.int32)
+    p.execute(vo, io, d)
+    radik.synchronize()
+    v, i = vo.to_host(), io.to_host()
+    off = oo = 0
+    for L, k in zip(lens, ks):
+        ev, ei = O.topk_exact(xs[off:off + L], [L], k, O.LARGEST | O.ASCEND)
+        assert np.array_equal(v[oo:oo + k].view(np.uint32), ev[0].view(np.uint32)), (L, k)
+        assert np.array_equal(i[oo:oo + k], ei[0]), (L, k)
+        off += L
+        oo += k
