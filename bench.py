@@ -69,13 +69,15 @@ def dominant_roofline(plan, phase_ms, steps, N, out_pairs, idx_bytes):
                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic_from_profiles(name)}
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def traffic_from_profiles(kernel, workload="uneven"):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (scaled by the copy-kernel
+    calibration), if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(kernel)
+        return t.get(workload, {}).get(kernel)
     except (OSError, ValueError):
         return None
 
@@ -216,6 +218,88 @@ def tune_all(radik, out_path):
     print(json.dumps(table), flush=True)
 
 
+def spiky_rows(rng, rows, cols):
+    """N(0, 3) rows with 0-4 entries per row set to exactly +20."""
+    m = (rng.standard_normal(rows * cols, dtype=np.float32) * np.float32(3)).astype(np.float32)
+    nsp = rng.integers(0, 5, rows)
+    r = np.repeat(np.arange(rows), nsp)
+    m[r * cols + rng.integers(0, cols, r.size)] = np.float32(20)
+    return m
+
+
+def serving(radik, seconds=3.0, seed=5150, pool_size=12, budget=16 << 30, inflight=4):
+    """Poisson arrivals of rows x 128256 requests (rows U{1..256}, k in {1, 50,
+    1024}), served (a) by one synchronous drop-in call per request in arrival
+    order and (b) by the continuous-batching server, at the same frozen rate:
+    lambda = 0.7 / (mean plain service time).  Latency = arrival -> results
+    complete; p50/p99 and rows/s for both."""
+    from radik import DeviceArray
+    cols = 128256
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(pool_size):
+        rows = int(rng.integers(1, 257))
+        k = int(rng.choice([1, 50, 1024]))
+        x = DeviceArray.from_host(spiky_rows(rng, rows, cols))
+        pool.append((rows, k, x, DeviceArray.zeros((k, rows), np.float32), DeviceArray.zeros((k, rows), np.int32)))
+    ws = radik.RadiKWorkspace(cols, 256)
+
+    def plain(req):
+        rows, k, x, vo, io = req
+        radik.topk_radix_select_(vo, io, ws, x, None, [cols] * rows, k, LARGEST=True, ASCEND=False)
+
+    for req in pool:  # warm and calibrate
+        plain(req)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        for req in pool:
+            plain(req)
+    s_mean = (time.perf_counter() - t0) / (reps * len(pool))
+    lam = 0.7 / s_mean
+    n_req = max(50, int(seconds * lam))
+    arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / lam, n_req))
+    which = np.random.default_rng(seed + 2).integers(0, len(pool), n_req)
+    rows_total = int(sum(pool[w][0] for w in which))
+
+    def wait_until(t):
+        while time.perf_counter() < t:
+            pass
+
+    lat_plain = np.empty(n_req)
+    t0 = time.perf_counter()
+    for i in range(n_req):
+        wait_until(t0 + arr[i])
+        plain(pool[which[i]])
+        lat_plain[i] = time.perf_counter() - (t0 + arr[i])
+    span_plain = time.perf_counter() - t0
+
+    srv = radik.Server(budget_bytes=budget, max_inflight=inflight, largest=True, rev=True)
+    for req in pool:  # warm the slots' buffers
+        srv.wait(srv.submit(req[2], req[0], cols, req[1], req[3], req[4]))
+    tickets = np.empty(n_req, dtype=np.int64)
+    lag = np.empty(n_req)
+    t0 = time.perf_counter()
+    for i in range(n_req):
+        wait_until(t0 + arr[i])
+        lag[i] = time.perf_counter() - (t0 + arr[i])  # submit delay behind the arrival
+        rows, k, x, vo, io = pool[which[i]]
+        tickets[i] = srv.submit(x, rows, cols, k, vo, io)
+    lat_srv = np.array([srv.wait(int(t)) for t in tickets]) * 1e-3 + lag
+    span_srv = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)
+    return {
+        "lambda_req_s": round(lam, 1), "requests": n_req, "plain_service_ms": round(s_mean * 1e3, 4),
+        "plain": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
+                  "rows_s": round(rows_total / span_plain, 1)},
+        "served": {"p50_ms": pct(lat_srv, 50), "p99_ms": pct(lat_srv, 99),
+                   "rows_s": round(rows_total / span_srv, 1), "launches": st["launches"],
+                   "max_coalesced": st["max_coalesced"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +310,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--only-batched", action="store_true",
                     help="time the batched plan only (profiling runs)")
+    ap.add_argument("--workload", default="uneven", choices=("uneven", "single"),
+                    help="batched-plan workload: the uneven batch, or one 2^28 task (profiling runs)")
+    ap.add_argument("--k", type=int, default=256)
     ap.add_argument("--tune", default="",
                     help="run the autotuner on the bench workloads and write the table to this path")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -257,11 +344,15 @@ def main():
         tune_all(radik, args.tune)
         return
     t0 = time.perf_counter()
-    x, lens = uneven_batch()
+    if args.workload == "single":
+        x = np.random.default_rng(2828).standard_normal(1 << 28, dtype=np.float32)
+        lens = np.array([1 << 28])
+    else:
+        x, lens = uneven_batch()
     N = int(lens.sum())
     T = len(lens)
-    k = 256
-    log(f"uneven batch: T={T} N={N} generated in {time.perf_counter() - t0:.1f}s")
+    k = args.k
+    log(f"{args.workload}: T={T} N={N} generated in {time.perf_counter() - t0:.1f}s")
     d_x = DeviceArray.from_host(x)
     vo = DeviceArray.zeros((k, T), np.float32)
     io = DeviceArray.zeros((k, T), np.int32)
@@ -274,7 +365,7 @@ def main():
     vh, ih = vo.to_host(), io.to_host()
     off = np.concatenate([[0], np.cumsum(lens)])
     check_ok = True
-    for t in (0, 1, T // 2, T - 1):
+    for t in sorted({0, min(1, T - 1), T // 2, T - 1}):
         ev, ei = O.topk_exact(x[off[t]:off[t + 1]], [lens[t]], k, O.LARGEST | O.ASCEND)
         check_ok &= bool(np.array_equal(vh[:, t].view(np.uint32), ev[0].view(np.uint32)) and
                          np.array_equal(ih[:, t], ei[0]))
@@ -312,6 +403,9 @@ def main():
 
     if not args.no_secondary:
         extra["secondary"] = secondary(radik, O, args.warmup, args.steps)
+        log("serving")
+        extra["secondary"]["serving_rows128256"] = serving(radik)
+        log(extra["secondary"]["serving_rows128256"])
 
     line = {
         "metric": "Gelem/s (batched top-k, uneven batch) + % HBM peak",
@@ -326,8 +420,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded numpy, generated on host, resident in HBM before timing)",
-        "config": {"workload": "uneven batch: 4096 tasks, len log-uniform [2^10,2^20] (seed 4096), k=256, "
-                               "largest, rev=false, Int32 indices, persistent batched plan (hipGraph)",
+        "config": {"workload": ("uneven batch: 4096 tasks, len log-uniform [2^10,2^20] (seed 4096), "
+                                f"k={k}, largest, rev=false, Int32 indices, persistent batched plan (hipGraph)"
+                                if args.workload == "uneven" else
+                                f"single task: 2^28 N(0,1) (seed 2828), k={k}, largest, rev=false, "
+                                "persistent batched plan (hipGraph)"),
                    "total_elements": N, "num_tasks": T, "k": k, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
