@@ -69,13 +69,15 @@ def dominant_roofline(plan, phase_ms, steps, N, out_pairs, idx_bytes):
                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic_from_profiles(name)}
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def traffic_from_profiles(kernel, workload="uneven"):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (scaled by the copy-kernel
+    calibration), if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(kernel)
+        return t.get(workload, {}).get(kernel)
     except (OSError, ValueError):
         return None
 
