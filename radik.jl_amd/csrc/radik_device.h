@@ -46,9 +46,11 @@ struct TaskCtl {
   uint32_t win_ctr;   // slot counter for winners (compaction)
   uint32_t cand_ctr;  // entries in the task's spill region
   uint32_t b_pred;    // predicted bound: pass A spills first digits <= b_pred (NO_BIN: no spill)
-  uint32_t spill_all; // 1: the spill holds winners and candidates; 0: candidates only
-  uint32_t fallback;  // 1: pass B re-reads the task (prediction missing or too tight)
-  uint32_t pad[5];
+  uint32_t spill_all; // 1: the spill holds winners and candidates; 0: pass B re-reads the task
+  uint32_t fin;       // 1: k_split compacts winners and candidates, k_sort finishes the task
+  uint32_t cc_ctr;    // candidates compacted behind the spill by k_split
+  uint64_t c_and;     // bitwise AND of the spilled candidates (k_split)
+  uint64_t c_or;      // bitwise OR of the spilled candidates (k_split)
 };
 static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 

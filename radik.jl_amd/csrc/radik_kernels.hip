@@ -199,13 +199,14 @@ constexpr int SAMPLE_GROUPS_PER_BLOCK = 16;
 
 __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ tasks,
                                                   const int64_t* __restrict__ sbases, int ntasks,
-                                                  int64_t total_groups, uint32_t* __restrict__ shist) {
+                                                  int64_t total_groups, uint32_t* __restrict__ shist,
+                                                  const uint32_t* __restrict__ bmap) {
   __shared__ uint32_t lh[HIST_BINS];
   for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
   int64_t g0 = (int64_t)blockIdx.x * SAMPLE_GROUPS_PER_BLOCK;
   const int64_t g1 = min(g0 + (int64_t)SAMPLE_GROUPS_PER_BLOCK, total_groups);
   if (g0 >= g1) return;
-  int t = find_task(sbases, ntasks, g0);
+  int t = (int)bmap[blockIdx.x];  // = find_task(sbases, ntasks, g0), from the host
   __syncthreads();
   while (true) {
     const TaskDesc d = tasks[t];
@@ -214,9 +215,19 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
     const int64_t se = min(g1, tb + G);
     const float sc = task_scaler(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-    for (int64_t g = g0; g < se; ++g) {
+    // all of this task's groups of the block in flight at once
+    float xv[SAMPLE_GROUPS_PER_BLOCK];
+#pragma unroll
+    for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
+      const int64_t g = g0 + u;
       const int64_t i = (g - tb) * d.len / G + threadIdx.x;
-      if (i < d.len) atomicAdd(&lh[rank_key(d.in[i], sc, xm) >> 20], 1u);
+      xv[u] = (g < se && i < d.len) ? d.in[i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
+      const int64_t g = g0 + u;
+      const int64_t i = (g - tb) * d.len / G + threadIdx.x;
+      if (g < se && i < d.len) atomicAdd(&lh[rank_key(xv[u], sc, xm) >> 20], 1u);
     }
     __syncthreads();
     flush_hist(lh, shist + (size_t)t * HIST_BINS);
@@ -302,7 +313,8 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
                                                 int64_t total, int64_t chunk,
                                                 uint32_t* __restrict__ hist,
                                                 TaskCtl* __restrict__ ctl,
-                                                uint64_t* __restrict__ cand) {
+                                                uint64_t* __restrict__ cand,
+                                                const uint32_t* __restrict__ bmap) {
   __shared__ uint32_t lh[HIST_BINS];
   __shared__ uint64_t stage_buf[BLOCK / 64][STAGE];
   __shared__ uint32_t stage_n[BLOCK / 64];
@@ -312,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
   int64_t g0 = (int64_t)blockIdx.x * chunk;
   const int64_t g1 = min(g0 + chunk, total);
   if (g0 >= g1) return;
-  int t = find_task(bases, ntasks, g0);
+  int t = (int)bmap[blockIdx.x];  // = find_task(bases, ntasks, g0), from the host
   __syncthreads();
   while (true) {
     const TaskDesc d = tasks[t];
@@ -351,7 +363,7 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ tasks,
                                                   uint32_t* __restrict__ hist,
-                                                  TaskCtl* __restrict__ ctl) {
+                                                  TaskCtl* __restrict__ ctl, uint32_t fin_cap) {
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
   const int t = blockIdx.x;
@@ -405,19 +417,26 @@ __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ t
     // Highest first digit the task needs: the candidate bin, or the last
     // winner bin when the threshold bin is taken whole.
     const uint32_t need = (o.b_cand != NO_BIN) ? o.b_cand : (o.b_win - 1);
+    // Few enough winners + candidates: k_sort ranks both and finishes the
+    // task from the spill region (behind the spill, or at its start after
+    // pass B).
+    const uint64_t both = (uint64_t)o.nwin + o.ncand;
+    o.fin = 0;
     if (o.m == 0) {
-      o.fallback = 0;
       o.spill_all = 1;
       o.cand_ctr = 0;
     } else if (prev.b_pred == NO_BIN || need > prev.b_pred || prev.cand_ctr > (uint32_t)len) {
-      o.fallback = 1;
       o.spill_all = 0;
       o.cand_ctr = 0;
+      o.fin = (both <= fin_cap && both <= (uint64_t)len) ? 1u : 0u;
     } else {
-      o.fallback = 0;
       o.spill_all = 1;
       o.cand_ctr = prev.cand_ctr;
+      o.fin = (both <= fin_cap && (uint64_t)o.cand_ctr + both <= (uint64_t)len) ? 1u : 0u;
     }
+    o.cc_ctr = 0;
+    o.c_and = ~0ull;
+    o.c_or = 0ull;
     ctl[t] = o;
   }
 }
@@ -434,19 +453,27 @@ __device__ __forceinline__ void pass_b_segment(const TaskDesc& d, int64_t s, int
                                                const WaveStage& scd) {
   const uint32_t bw = c.b_win, bc = c.b_cand, nwin = c.nwin;
   const uint32_t rmax = digit_bound(bc != NO_BIN ? bc : bw - 1);
-  const uint32_t clen = (uint32_t)d.len;
+  // fin: winners (as selection composites) and candidates go to the spill
+  // region, winners first, for k_sort; else winners to the output slots.
+  const bool fin = c.fin != 0;
+  const uint32_t clen = fin ? c.ncand : (uint32_t)d.len;
+  uint64_t* __restrict__ cdst = cb + (fin ? nwin : 0u);
   auto write_win = [&](uint32_t slot, uint64_t v) {
-    if (slot < nwin) put_result(d, slot, __uint_as_float((uint32_t)(v >> 32)), (uint32_t)v);
+    if (slot < nwin) {
+      if (fin) cb[slot] = v;
+      else put_result(d, slot, __uint_as_float((uint32_t)(v >> 32)), (uint32_t)v);
+    }
   };
   auto write_cand = [&](uint32_t slot, uint64_t v) {
-    if (slot < clen) cb[slot] = v;
+    if (slot < clen) cdst[slot] = v;
   };
   stream_range<UNR>(d.in, s, e, [&](float x, uint32_t pos) {
     const uint32_t rk = rk_of<SCALE>(x, sc, xm);
     const bool hit = rk <= rmax;
     if (ballot(hit)) {
       const uint32_t dg = rk >> 20;
-      stage_append(sw, hit && dg < bw, ((uint64_t)__float_as_uint(x) << 32) | pos, wctr, write_win);
+      const uint64_t wv = ((uint64_t)(fin ? rk : __float_as_uint(x)) << 32) | pos;
+      stage_append(sw, hit && dg < bw, wv, wctr, write_win);
       stage_append(scd, hit && dg == bc, ((uint64_t)rk << 32) | pos, cctr, write_cand);
     }
   });
@@ -459,7 +486,8 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
                                                    const int64_t* __restrict__ bases, int ntasks,
                                                    int64_t g_begin, int64_t total, int64_t chunk,
                                                    TaskCtl* __restrict__ ctl,
-                                                   uint64_t* __restrict__ cand) {
+                                                   uint64_t* __restrict__ cand,
+                                                   const uint32_t* __restrict__ bmap) {
   __shared__ uint64_t stage_w[BLOCK / 64][STAGE];
   __shared__ uint64_t stage_c[BLOCK / 64][STAGE];
   __shared__ uint32_t stage_n[2][BLOCK / 64];
@@ -470,13 +498,13 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
   int64_t g0 = g_begin + (int64_t)blockIdx.x * chunk;
   const int64_t g1 = min(g0 + chunk, total);
   if (g0 >= g1) return;
-  int t = find_task(bases, ntasks, g0);
+  int t = (int)bmap[blockIdx.x];  // = find_task(bases, ntasks, g0), from the host
   while (true) {
     const TaskDesc d = tasks[t];
     const int64_t tb = bases[t];
     const int64_t se = min(g1, tb + d.len);
     const TaskCtl c = ctl[t];
-    if (c.fallback && c.m > 0) {
+    if (!c.spill_all && c.m > 0) {
       const float sc = task_scaler(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
       uint32_t* wctr = &ctl[t].win_ctr;
@@ -496,6 +524,48 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
 // block keeps enough requests in flight when the candidates sit in global
 // memory.
 constexpr int LOADS_IN_FLIGHT = 8;
+
+struct LdsSrc {
+  const uint64_t* p;
+  __device__ uint64_t operator[](uint32_t i) const { return p[i]; }
+};
+struct GlobalSrc {
+  const uint64_t* __restrict__ p;
+  __device__ uint64_t operator[](uint32_t i) const { return p[i]; }
+};
+
+// Global candidates: 16-byte loads (two entries), GLOBAL_LOADS in flight per
+// thread, so one block keeps 64 KiB of requests outstanding.  A head entry
+// before 16-byte alignment and an odd tail entry go to thread 0.
+constexpr int GLOBAL_LOADS = 16;
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+template <typename F>
+__device__ __forceinline__ void for_each_entry(GlobalSrc src, uint32_t n, F&& f) {
+  const uint64_t* __restrict__ p = src.p;
+  const uint32_t head = (((uintptr_t)p & 15u) != 0 && n > 0) ? 1u : 0u;
+  if (head && threadIdx.x == 0) f(p[0]);
+  const uint64_t* __restrict__ q = p + head;
+  const uint32_t m = n - head;
+  const uint32_t npair = m >> 1;
+  const u64x2* __restrict__ v = reinterpret_cast<const u64x2*>(q);
+  for (uint32_t b = threadIdx.x; b < npair; b += BLOCK * GLOBAL_LOADS) {
+    u64x2 xs[GLOBAL_LOADS];
+#pragma unroll
+    for (int u = 0; u < GLOBAL_LOADS; ++u) {
+      const uint32_t i = b + (uint32_t)u * BLOCK;
+      xs[u] = (i < npair) ? v[i] : u64x2{0ull, 0ull};
+    }
+#pragma unroll
+    for (int u = 0; u < GLOBAL_LOADS; ++u) {
+      if (b + (uint32_t)u * BLOCK < npair) {
+        f((uint64_t)xs[u].x);
+        f((uint64_t)xs[u].y);
+      }
+    }
+  }
+  if ((m & 1u) && threadIdx.x == 0) f(q[m - 1]);
+}
 
 template <typename Src, typename F>
 __device__ __forceinline__ void for_each_entry(Src src, uint32_t n, F&& f) {
@@ -524,16 +594,21 @@ __device__ __forceinline__ void for_each_entry(Src src, uint32_t n, F&& f) {
 // differ and `same` holds their common bits; a digit without varying bits has
 // a single bin and is fixed without a pass (exact ties skip the rank digits).
 // ---------------------------------------------------------------------------
-template <typename Src>
+template <bool NARROW = false, typename Src>
 __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint64_t& P, uint32_t need,
                                              uint64_t vary, uint64_t same, uint32_t* lh, uint32_t* wsum,
                                              uint32_t* res) {
-  const int shifts[6] = {52, 40, 32, 20, 8, 0};
-  const int widths[6] = {12, 12, 8, 12, 12, 8};
+  // Wide digits (4096-bin histograms) for large sets; narrow ones (at most
+  // 256 bins, top <= 52) for small sets already in LDS.
+  constexpr int ND = NARROW ? 7 : 6;
+  constexpr int shifts_w[6] = {52, 40, 32, 20, 8, 0};
+  constexpr int widths_w[6] = {12, 12, 8, 12, 12, 8};
+  constexpr int shifts_n[7] = {44, 36, 32, 24, 16, 8, 0};
+  constexpr int widths_n[7] = {8, 8, 4, 8, 8, 8, 8};
   int s_final = top;
-  for (int q = 0; q < 6; ++q) {
-    const int s = shifts[q];
-    const int w = widths[q];
+  for (int q = 0; q < ND; ++q) {
+    const int s = NARROW ? shifts_n[q] : shifts_w[q];
+    const int w = NARROW ? widths_n[q] : widths_w[q];
     if (s + w > top) continue;  // digit already fixed
     const uint32_t nb = 1u << w;
     const int hs = s + w;
@@ -557,7 +632,7 @@ __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint6
       block_find_bin<16>(c, need, wsum, res);
     } else {
       uint32_t c[1];
-      c[0] = lh[threadIdx.x];
+      c[0] = (threadIdx.x < nb) ? lh[threadIdx.x] : 0u;
       block_find_bin<1>(c, need, wsum, res);
     }
     const uint32_t b = res[0], before = res[1], cnt = res[2];
@@ -583,14 +658,6 @@ __device__ __forceinline__ void block_and_or(uint64_t a, uint64_t o, unsigned lo
 // spills), then resolves the candidates of the threshold bin digit by digit
 // (in LDS when they fit) and emits the chosen ones.
 // ---------------------------------------------------------------------------
-struct LdsSrc {
-  const uint64_t* p;
-  __device__ uint64_t operator[](uint32_t i) const { return p[i]; }
-};
-struct GlobalSrc {
-  const uint64_t* __restrict__ p;
-  __device__ uint64_t operator[](uint32_t i) const { return p[i]; }
-};
 
 // Value of a spilled composite: the original element.
 __device__ __forceinline__ float spilled_value(const TaskDesc& d, uint64_t x, uint32_t& pos) {
@@ -611,9 +678,11 @@ __device__ __forceinline__ void refine_emit(const TaskDesc& d, const TaskCtl& c,
   const uint32_t lim = c.m - c.nwin;
   const uint64_t bc = c.b_cand;
   for_each_entry(src, n, [&](uint64_t x) {
-    if ((x >> 52) == bc && (x >> s) <= P) {
-      const uint32_t slot = atomicAdd(ectr, 1u);
-      if (slot < lim) {
+    const bool take = (x >> 52) == bc && (x >> s) <= P;
+    const uint64_t tm = ballot(take);
+    if (tm) {
+      const uint32_t slot = wave_append(take, tm, ectr);
+      if (take && slot < lim) {
         uint32_t pos;
         const float v = spilled_value(d, x, pos);
         put_result(d, c.nwin + slot, v, pos);
@@ -622,6 +691,120 @@ __device__ __forceinline__ void refine_emit(const TaskDesc& d, const TaskCtl& c,
   });
 }
 
+// Candidates of a spilled task are compacted behind its spill, at [n, n +
+// ncand) of the spill region (n = spilled entries), when that stays inside
+// the region; otherwise k_refine reads them in place among the spill.
+__device__ __forceinline__ bool cands_compacted(const TaskCtl& c, int64_t len) {
+  return c.spill_all && c.krem > 0 && (uint64_t)c.cand_ctr + c.ncand <= (uint64_t)len;
+}
+
+// ---------------------------------------------------------------------------
+// Split of the pass-A spills, several blocks per predicted task (xmap: task
+// of each block, xbases: prefix sums of the blocks per task; the predicted
+// tasks come first).  Each
+// block takes an equal share of its task's spill: winners (first digit <
+// b_win) go to the output slots [0, nwin), candidates (digit == b_cand) are
+// compacted behind the spill, and the candidates' bitwise AND / OR are
+// merged into the control block for k_refine's digit skipping.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ tasks,
+                                                 const int64_t* __restrict__ xbases,
+                                                 const uint32_t* __restrict__ xmap,
+                                                 TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand) {
+  constexpr int U = LOADS_IN_FLIGHT;
+  __shared__ unsigned long long vb[2];
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t slot0[2];
+  const uint32_t t = xmap[blockIdx.x];
+  const TaskCtl c = ctl[t];
+  if (!c.spill_all || c.m == 0) return;
+  const TaskDesc d = tasks[t];
+  uint32_t n = c.cand_ctr;
+  if (n > (uint32_t)d.len) n = (uint32_t)d.len;
+  const uint32_t nb = (uint32_t)(xbases[t + 1] - xbases[t]);
+  const uint32_t j = (uint32_t)(blockIdx.x - xbases[t]);
+  const uint32_t per = (n + nb - 1) / nb;
+  const uint32_t s = min(n, j * per), e = min(n, s + per);
+  if (s >= e) return;
+  // fin: winners and candidates both go behind the spill (winners first);
+  // else winners to the output slots and candidates behind the spill if they fit.
+  const bool fin = c.fin != 0;
+  const bool compact = fin || cands_compacted(c, d.len);
+  uint64_t* __restrict__ spill = cand + d.base;
+  uint64_t* __restrict__ cdst = spill + n + (fin ? c.nwin : 0u);
+  if (threadIdx.x == 0) {
+    vb[0] = ~0ull;
+    vb[1] = 0ull;
+  }
+  uint64_t a = ~0ull, o = 0ull;
+  const uint32_t bw = c.b_win, bc = c.b_cand;
+  // Tiles of BLOCK * U entries: one block scan and one pair of global atomics
+  // place all of a tile's winners and candidates.
+  for (uint32_t tile = s; tile < e; tile += (uint32_t)BLOCK * U) {
+    uint64_t xs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = tile + (uint32_t)u * BLOCK + threadIdx.x;
+      xs[u] = (i < e) ? spill[i] : 0ull;
+    }
+    uint32_t wmask = 0, cmask = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (tile + (uint32_t)u * BLOCK + threadIdx.x >= e) continue;
+      const uint32_t dg = (uint32_t)(xs[u] >> 52);
+      if (dg < bw) {
+        wmask |= 1u << u;
+      } else if (dg == bc) {
+        cmask |= 1u << u;
+        a &= xs[u];
+        o |= xs[u];
+      }
+    }
+    const uint32_t mine = (uint32_t)__builtin_popcount(wmask) | ((compact ? (uint32_t)__builtin_popcount(cmask) : 0u) << 16);
+    const uint32_t incl = block_incl_scan(mine, wsum);
+    if (threadIdx.x == BLOCK - 1) {
+      const uint32_t tw = incl & 0xFFFFu, tc = incl >> 16;
+      slot0[0] = tw ? atomicAdd(&ctl[t].win_ctr, tw) : 0u;
+      slot0[1] = tc ? atomicAdd(&ctl[t].cc_ctr, tc) : 0u;
+    }
+    __syncthreads();
+    const uint32_t excl = incl - mine;
+    uint32_t ws = slot0[0] + (excl & 0xFFFFu), cs = slot0[1] + (excl >> 16);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (wmask & (1u << u)) {
+        if (ws < c.nwin) {
+          if (fin) {
+            spill[n + ws] = xs[u];
+          } else {
+            uint32_t pos;
+            const float v = spilled_value(d, xs[u], pos);
+            put_result(d, ws, v, pos);
+          }
+        }
+        ++ws;
+      } else if (compact && (cmask & (1u << u))) {
+        if (cs < c.ncand) cdst[cs] = xs[u];
+        ++cs;
+      }
+    }
+  }
+  if (c.krem == 0 || fin) return;
+  block_and_or(a, o, vb);
+  // (any candidate leaves the AND below all-ones or the OR above zero)
+  if (threadIdx.x == 0 && (vb[0] != ~0ull || vb[1] != 0ull)) {
+    atomicAnd((unsigned long long*)&ctl[t].c_and, vb[0]);
+    atomicOr((unsigned long long*)&ctl[t].c_or, vb[1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Refinement, one block per task with candidates left: resolves the
+// threshold bin's candidates digit by digit (in LDS when they fit) and emits
+// the chosen ones behind the winners.  The candidates are the compacted ones
+// of k_split, the spill itself when they could not be compacted, or pass B's
+// candidate list.
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ tasks,
                                                   const TaskCtl* __restrict__ ctl,
                                                   const uint64_t* __restrict__ cand,
@@ -630,71 +813,52 @@ __global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ t
   __shared__ uint32_t lh[HIST_BINS];
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
-  __shared__ uint32_t ectr, wctr, nlds;
+  __shared__ uint32_t ectr;
   __shared__ unsigned long long vb[2];
   const int t = blockIdx.x;
   const TaskCtl c = ctl[t];
-  if (c.m == 0 || (!c.spill_all && c.krem == 0)) return;
+  if (c.m == 0 || c.krem == 0 || c.fin) return;
   const TaskDesc d = tasks[t];
   uint32_t n = c.cand_ctr;
   if (n > (uint32_t)d.len) n = (uint32_t)d.len;
+  const uint64_t* __restrict__ src = cand + d.base;
+  if (cands_compacted(c, d.len)) {
+    src += n;
+    n = min(c.cc_ctr, c.ncand);
+  }
+  // Spilled tasks bring the AND / OR from k_split; pass B's lists get them here.
+  const bool have_vb = c.spill_all != 0;
   if (threadIdx.x == 0) {
     ectr = 0;
-    wctr = 0;
-    nlds = 0;
-    vb[0] = ~0ull;
-    vb[1] = 0ull;
+    vb[0] = have_vb ? (unsigned long long)c.c_and : ~0ull;
+    vb[1] = have_vb ? (unsigned long long)c.c_or : 0ull;
   }
   __syncthreads();
-  const uint64_t* __restrict__ src = cand + d.base;
-  const uint64_t bc = c.b_cand;
-  uint64_t a = ~0ull, o = 0ull;  // AND / OR of this thread's candidates
-  if (c.spill_all) {
-    // Split the spill: winners to the output slots, candidates to LDS.
-    const bool to_lds = c.krem > 0 && c.ncand <= lds_cap;
-    for_each_entry(GlobalSrc{src}, n, [&](uint64_t x) {
-      const uint32_t dg = (uint32_t)(x >> 52);
-      if (dg < c.b_win) {
-        const uint32_t slot = atomicAdd(&wctr, 1u);
-        if (slot < c.nwin) {
-          uint32_t pos;
-          const float v = spilled_value(d, x, pos);
-          put_result(d, slot, v, pos);
-        }
-      } else if (dg == bc) {
-        a &= x;
-        o |= x;
-        if (to_lds) {
-          const uint32_t j = atomicAdd(&nlds, 1u);
-          if (j < lds_cap) cbuf[j] = x;
+  uint64_t a = ~0ull, o = 0ull;
+  const bool in_lds = n <= lds_cap;
+  if (in_lds || !have_vb) {
+    for (uint32_t b = threadIdx.x; b < n; b += BLOCK * LOADS_IN_FLIGHT) {
+      uint64_t xs[LOADS_IN_FLIGHT];
+#pragma unroll
+      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+        const uint32_t i = b + (uint32_t)u * BLOCK;
+        xs[u] = (i < n) ? src[i] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+        const uint32_t i = b + (uint32_t)u * BLOCK;
+        if (i < n) {
+          if (in_lds) cbuf[i] = xs[u];
+          a &= xs[u];
+          o |= xs[u];
         }
       }
-    });
-    __syncthreads();
-    if (c.krem == 0) return;
-    block_and_or(a, o, vb);
-    if (to_lds) {
-      const uint32_t nc = nlds < lds_cap ? nlds : lds_cap;
-      refine_emit(d, c, LdsSrc{cbuf}, nc, vb, lh, wsum, res, &ectr);
-    } else {
-      refine_emit(d, c, GlobalSrc{src}, n, vb, lh, wsum, res, &ectr);
     }
-  } else if (n <= lds_cap) {
-    for_each_entry(GlobalSrc{src}, n, [&](uint64_t x) {
-      a &= x;
-      o |= x;
-    });
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) cbuf[i] = src[i];
-    block_and_or(a, o, vb);
-    refine_emit(d, c, LdsSrc{cbuf}, n, vb, lh, wsum, res, &ectr);
-  } else {
-    for_each_entry(GlobalSrc{src}, n, [&](uint64_t x) {
-      a &= x;
-      o |= x;
-    });
-    block_and_or(a, o, vb);
-    refine_emit(d, c, GlobalSrc{src}, n, vb, lh, wsum, res, &ectr);
   }
+  if (!have_vb) block_and_or(a, o, vb);
+  else __syncthreads();
+  if (in_lds) refine_emit(d, c, LdsSrc{cbuf}, n, vb, lh, wsum, res, &ectr);
+  else refine_emit(d, c, GlobalSrc{src}, n, vb, lh, wsum, res, &ectr);
 }
 
 // ---------------------------------------------------------------------------
@@ -874,10 +1038,139 @@ __device__ __forceinline__ void sort_task_regs(const TaskDesc& d, uint32_t m, ui
   for (uint32_t i = (uint32_t)BLOCK * E + t; i < k; i += BLOCK) write_slot(d, i, 0ull, false);
 }
 
+// Output composite of a selection composite (rank key << 32 | position).
+__device__ __forceinline__ uint64_t out_of_selection(const TaskDesc& d, uint64_t x) {
+  uint32_t pos;
+  const float v = spilled_value(d, x, pos);
+  return out_composite(d, v, pos);
+}
+
+// Selection and output orders agree (no scaling; LARGEST with descending
+// output, or smallest with ascending output): then the composites are equal.
+__device__ __forceinline__ bool same_order(const TaskDesc& d) {
+  const bool largest = (d.flags & F_LARGEST) != 0, ascend = (d.flags & F_ASCEND) != 0;
+  return !(d.flags & F_WITHSCALE) && largest != ascend;
+}
+
+// Sorts the m selection composites sel[0, m) (LDS) by output order in
+// registers and writes the k output slots (padding past m).  xs: LDS scratch
+// of BLOCK * E entries.
+template <int E>
+__device__ __forceinline__ void sort_selected_regs(const TaskDesc& d, const uint64_t* sel, uint32_t m, uint64_t* xs) {
+  const uint32_t t = threadIdx.x;
+  const bool same = same_order(d);
+  uint64_t a[E];
+#pragma unroll
+  for (int l = 0; l < E; ++l) {
+    const uint32_t i = t * E + l;
+    a[l] = (i < m) ? (same ? sel[i] : out_of_selection(d, sel[i])) : ~0ull;
+  }
+  reg_bitonic<E>(a, xs);
+  const uint32_t k = (uint32_t)d.k;
+#pragma unroll
+  for (int l = 0; l < E; ++l) {
+    const uint32_t i = t * E + l;
+    if (i < k) write_slot(d, i, a[l], i < m);
+  }
+  for (uint32_t i = (uint32_t)BLOCK * E + t; i < k; i += BLOCK) write_slot(d, i, 0ull, false);
+}
+
+template <int EMAX>
+__device__ __forceinline__ void sort_selected(const TaskDesc& d, const uint64_t* sel, uint32_t m, uint64_t* xs) {
+  if constexpr (EMAX > 1) {
+    if (m <= (uint32_t)BLOCK * (EMAX / 2)) {
+      sort_selected<EMAX / 2>(d, sel, m, xs);
+      return;
+    }
+  }
+  sort_selected_regs<EMAX>(d, sel, m, xs);
+}
+
+// LDS of a finishing block beside the dynamic arrays.
+struct FinLds {
+  uint32_t lh[256];
+  uint32_t wsum[BLOCK / 64];
+  uint32_t res[3];
+  uint32_t ectr;
+  unsigned long long vb[2];
+};
+
+// Finishes a task whose nwin winners and ncand candidates lie compacted at
+// src (winners first): loads them to v (LDS), resolves the krem best
+// candidates with narrow digits, gathers winners and chosen candidates in
+// sel and sorts those by output order.  fin_cap entries of v; EMAX covers
+// fin_cap.
+template <int EMAX>
+__device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c, const uint64_t* __restrict__ src,
+                                            uint64_t* v, uint64_t* sel, FinLds& L) {
+  const uint32_t nwin = c.nwin, ncand = c.ncand, cnt = nwin + ncand, krem = c.krem;
+  if (threadIdx.x == 0) {
+    L.ectr = 0;
+    L.vb[0] = ~0ull;
+    L.vb[1] = 0ull;
+  }
+  uint64_t a = ~0ull, o = 0ull;
+  for (uint32_t b = threadIdx.x; b < cnt; b += BLOCK * LOADS_IN_FLIGHT) {
+    uint64_t xs[LOADS_IN_FLIGHT];
+#pragma unroll
+    for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+      const uint32_t i = b + (uint32_t)u * BLOCK;
+      xs[u] = (i < cnt) ? src[i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+      const uint32_t i = b + (uint32_t)u * BLOCK;
+      if (i < cnt) {
+        v[i] = xs[u];
+        if (i >= nwin) {
+          a &= xs[u];
+          o |= xs[u];
+        }
+      }
+    }
+  }
+  __syncthreads();  // L initialised before the atomics below
+  block_and_or(a, o, L.vb);
+  uint64_t P = c.b_cand;
+  int s = 52;
+  if (krem > 0 && krem < ncand)
+    s = select_digits<true>(LdsSrc{v + nwin}, ncand, 52, P, krem, (uint64_t)(L.vb[0] ^ L.vb[1]),
+                            (uint64_t)L.vb[0], L.lh, L.wsum, L.res);
+  for (uint32_t i = threadIdx.x; i < nwin; i += BLOCK) sel[i] = v[i];
+  for (uint32_t i = threadIdx.x; i < ncand; i += BLOCK) {
+    const uint64_t x = v[nwin + i];
+    const bool take = (x >> s) <= P;
+    const uint64_t tm = ballot(take);
+    if (tm) {
+      const uint32_t slot = wave_append(take, tm, &L.ectr);
+      if (take && slot < krem) sel[nwin + slot] = x;
+    }
+  }
+  __syncthreads();  // sel complete; v is free for the sort's scratch
+  sort_selected<EMAX>(d, sel, c.m, v);
+}
+
+template <int EMAX>
+__device__ __forceinline__ void sort_any(const TaskDesc& d, uint32_t m, uint64_t* xs) {
+  if constexpr (EMAX > 1) {
+    if (m <= (uint32_t)BLOCK * (EMAX / 2)) {
+      sort_any<EMAX / 2>(d, m, xs);
+      return;
+    }
+  }
+  sort_task_regs<EMAX>(d, m, xs);
+}
+
+// Largest nwin + ncand finished by k_sort (a power of two >= BLOCK) and the
+// register blocking that sorts that many.
+constexpr uint32_t FIN_CAP = 2048;
+constexpr int FIN_E = (int)(FIN_CAP / BLOCK);
+
 // ---------------------------------------------------------------------------
-// Final sort of the staged results, one block per task: registers + LDS up to
-// reg_cap results, else a bitonic in the task's (now free) spill region,
-// which holds len >= m slots.
+// Final sort, one block per task.  Tasks finished here (fin) sort their
+// compacted winners and candidates; the others sort the m results staged in
+// their output slots: registers + LDS up to BLOCK * E results, else a bitonic
+// in the task's (now free) spill region, which holds len >= m slots.
 // ---------------------------------------------------------------------------
 struct GlobalPtr {
   uint64_t* __restrict__ p;
@@ -891,12 +1184,20 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
                                                 const TaskCtl* __restrict__ ctl,
                                                 uint64_t* __restrict__ cand) {
   extern __shared__ uint64_t sb[];
+  __shared__ FinLds fl;
   const int t = blockIdx.x;
   const TaskDesc d = tasks[t];
-  const uint32_t m = ctl[t].m;
+  const TaskCtl c = ctl[t];
+  const uint32_t m = c.m;
   const bool i64 = (d.flags & F_IDX64) != 0;
-  if (m <= (uint32_t)BLOCK * E) {
-    sort_task_regs<E>(d, m, sb);
+  if (c.fin) {
+    // k_select admits only nwin + ncand <= fin_cap; the dynamic LDS holds
+    // fin_cap entries of v, then sel
+    uint32_t n = c.spill_all ? c.cand_ctr : 0u;
+    if (n > (uint32_t)d.len) n = (uint32_t)d.len;
+    finish_task<FIN_E>(d, c, cand + d.base + n, sb, sb + FIN_CAP, fl);
+  } else if (m <= (uint32_t)BLOCK * E) {
+    sort_any<E>(d, m, sb);
   } else {
     uint64_t* __restrict__ g = cand + d.base;
     for (uint32_t i = threadIdx.x; i < m; i += BLOCK) {
@@ -1017,33 +1318,45 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
   if (n > cap) return;  // host routes only short tasks here
   const float sc = task_scaler(d);
   const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-  for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
-    all[i] = ((uint64_t)rank_key(d.in[i], sc, xm) << 32) | i;
   if (threadIdx.x == 0) {
     ectr = 0;
     vb[0] = ~0ull;
     vb[1] = 0ull;
   }
   __syncthreads();
+  uint64_t a = ~0ull, o = 0ull;
+  for (uint32_t b = threadIdx.x; b < n; b += BLOCK * LOADS_IN_FLIGHT) {
+    float xs[LOADS_IN_FLIGHT];
+#pragma unroll
+    for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+      const uint32_t i = b + (uint32_t)u * BLOCK;
+      xs[u] = (i < n) ? d.in[i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+      const uint32_t i = b + (uint32_t)u * BLOCK;
+      if (i < n) {
+        const uint64_t x = ((uint64_t)rank_key(xs[u], sc, xm) << 32) | i;
+        all[i] = x;
+        a &= x;
+        o |= x;
+      }
+    }
+  }
+  block_and_or(a, o, vb);  // also publishes `all`
   int s = 64;
   uint64_t P = 0;
-  if (m > 0 && m < n) {
-    uint64_t a = ~0ull, o = 0ull;
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-      a &= all[i];
-      o |= all[i];
-    }
-    block_and_or(a, o, vb);
+  if (m > 0 && m < n)
     s = select_digits(LdsSrc{all}, n, 64, P, m, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh, wsum, res);
-  }
   const bool scale = (d.flags & F_WITHSCALE) != 0;
   if (m > 0) {
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
       const uint64_t x = all[i];
       const bool take = (m >= n) ? true : ((x >> s) <= P);
-      if (take) {
-        const uint32_t slot = atomicAdd(&ectr, 1u);
-        if (slot < m) {
+      const uint64_t tm = ballot(take);
+      if (tm) {
+        const uint32_t slot = wave_append(take, tm, &ectr);
+        if (take && slot < m) {
           const uint32_t pos = (uint32_t)x;
           const float v = scale ? d.in[pos] : fkey_inv((uint32_t)(x >> 32) ^ xm);
           sel[slot] = out_composite(d, v, pos);
@@ -1084,9 +1397,9 @@ __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ s
 // Explicit instantiations used by the host.
 #define RADIK_INST(U)                                                                                   \
   template __global__ void k_hist<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, uint32_t*, \
-                                     TaskCtl*, uint64_t*);                                              \
+                                     TaskCtl*, uint64_t*, const uint32_t*);                             \
   template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, int64_t,   \
-                                        TaskCtl*, uint64_t*);
+                                        TaskCtl*, uint64_t*, const uint32_t*);
 RADIK_INST(1)
 RADIK_INST(2)
 RADIK_INST(4)

@@ -94,6 +94,12 @@ Pipeline::~Pipeline() {
   if (d_cbases) (void)hipFree(d_cbases);
   if (h_mbases) (void)hipHostFree(h_mbases);
   if (d_mbases) (void)hipFree(d_mbases);
+  if (h_xbases) (void)hipHostFree(h_xbases);
+  if (d_xbases) (void)hipFree(d_xbases);
+  if (h_xmap) (void)hipHostFree(h_xmap);
+  if (d_xmap) (void)hipFree(d_xmap);
+  if (h_bmap) (void)hipHostFree(h_bmap);
+  if (d_bmap) (void)hipFree(d_bmap);
   if (d_shist) (void)hipFree(d_shist);
   if (h_small) (void)hipHostFree(h_small);
   if (d_large) (void)hipFree(d_large);
@@ -103,6 +109,9 @@ Pipeline::~Pipeline() {
   if (d_hist) (void)hipFree(d_hist);
   if (d_cand) (void)hipFree(d_cand);
   if (upload_done) (void)hipEventDestroy(upload_done);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
 }
 
 static size_t grow(size_t have, size_t need) {
@@ -111,8 +120,13 @@ static size_t grow(size_t have, size_t need) {
   return c;
 }
 
-int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
+int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, int64_t nx, hipStream_t s) {
   if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
+  if (!side) {  // default priority, like every stream of this library
+    RADIK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    RADIK_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    RADIK_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  }
   // The pinned arrays may still feed an in-flight copy.
   RADIK_HIP(hipEventSynchronize(upload_done));
   if (!h_large || (size_t)nl > cap_large) {  // the prefix arrays always need their sentinel
@@ -124,20 +138,26 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
     if (h_sbases) RADIK_HIP(hipHostFree(h_sbases));
     if (h_cbases) RADIK_HIP(hipHostFree(h_cbases));
     if (h_mbases) RADIK_HIP(hipHostFree(h_mbases));
+    if (h_xbases) RADIK_HIP(hipHostFree(h_xbases));
     if (d_large) RADIK_HIP(hipFree(d_large));
     if (d_bases) RADIK_HIP(hipFree(d_bases));
     if (d_sbases) RADIK_HIP(hipFree(d_sbases));
     if (d_cbases) RADIK_HIP(hipFree(d_cbases));
     if (d_mbases) RADIK_HIP(hipFree(d_mbases));
+    if (d_xbases) RADIK_HIP(hipFree(d_xbases));
     if (d_ctl) RADIK_HIP(hipFree(d_ctl));
     h_large = nullptr; h_bases = nullptr; h_sbases = nullptr; h_cbases = nullptr; h_mbases = nullptr;
+    h_xbases = nullptr;
     d_large = nullptr; d_bases = nullptr; d_sbases = nullptr; d_cbases = nullptr; d_mbases = nullptr;
+    d_xbases = nullptr;
     d_ctl = nullptr;
     if (hipHostMalloc((void**)&h_large, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_bases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_sbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_cbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_mbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_xbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_xbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_cbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_mbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_large, c * sizeof(TaskDesc)) != hipSuccess ||
@@ -178,6 +198,19 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
     }
     cap_small = c;
   }
+  if (!h_xmap || (size_t)nx > cap_xmap) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = grow(cap_xmap, (size_t)nx);
+    if (h_xmap) RADIK_HIP(hipHostFree(h_xmap));
+    if (d_xmap) RADIK_HIP(hipFree(d_xmap));
+    h_xmap = nullptr; d_xmap = nullptr;
+    if (hipHostMalloc((void**)&h_xmap, c * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_xmap, c * sizeof(uint32_t)) != hipSuccess) {
+      cap_xmap = 0;
+      return set_error(6, "allocation of a %zu-entry split map failed", c);
+    }
+    cap_xmap = c;
+  }
   if ((size_t)tot > cap_cand) {
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = (size_t)tot;
@@ -205,6 +238,25 @@ static int64_t sample_groups(int64_t len, int64_t k) {
   return g;
 }
 
+static int64_t stream_chunk(int64_t total, const DevFacts& f, const LaunchOpts& o, int* grid);
+
+// First task (largest t with pre[t] <= g0) of each block b, g0 = gbeg + b * chunk.
+static void fill_block_map(uint32_t* out, int grid, int64_t gbeg, int64_t chunk, const int64_t* pre, int64_t nt) {
+  int64_t t = 0;
+  for (int b = 0; b < grid; ++b) {
+    const int64_t g0 = gbeg + (int64_t)b * chunk;
+    while (t + 1 < nt && pre[t + 1] <= g0) ++t;
+    out[b] = (uint32_t)t;
+  }
+}
+
+// k_split blocks of a predicted task: one per 2^17 elements, 1..128 (the
+// spill grows with the task and with ties at the threshold).
+static int64_t split_blocks(int64_t len) {
+  const int64_t b = (len + ((int64_t)1 << 17) - 1) >> 17;
+  return b < 1 ? 1 : (b > 128 ? 128 : b);
+}
+
 // Short tasks run whole in one block (k_small); small_cap 0 routes every
 // task through the streaming passes.
 static inline bool is_short(int64_t len, uint32_t small_cap) {
@@ -216,19 +268,22 @@ static inline bool is_short(int64_t len, uint32_t small_cap) {
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
                 hipStream_t s) {
   const uint32_t scap = eff_small_cap(f, o);
-  int64_t nl = 0, ns = 0, tot = 0, tot_pred = 0;
+  int64_t nl = 0, ns = 0, tot = 0, tot_pred = 0, nx = 0;
   for (int64_t t = 0; t < n; ++t) {
     if (is_short(tasks[t].len, scap)) {
       ++ns;
     } else {
       ++nl;
       tot += tasks[t].len;
-      if (o.predict && sample_groups(tasks[t].len, tasks[t].k)) tot_pred += tasks[t].len;
+      if (o.predict && sample_groups(tasks[t].len, tasks[t].k)) {
+        tot_pred += tasks[t].len;
+        nx += split_blocks(tasks[t].len);
+      }
     }
   }
-  int rc = p.reserve(nl, ns, tot, s);
+  int rc = p.reserve(nl, ns, tot, nx, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0, runs = 0, msum = 0;
+  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0, runs = 0, msum = 0, npred = 0, xsum = 0;
   // pass 0: predicted long tasks; pass 1: short tasks and the other long tasks
   for (int pass = 0; pass < 2; ++pass)
   for (int64_t t = 0; t < n; ++t) {
@@ -257,6 +312,13 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
       p.h_mbases[il] = msum;
       runs += std::max<int64_t>(1, (m + SORT_CHUNK - 1) / SORT_CHUNK);  // run 0 also pads
       msum += m;
+      if (pred) {
+        p.h_xbases[npred] = xsum;
+        const int64_t nb = split_blocks(ts.len);
+        for (int64_t j = 0; j < nb; ++j) p.h_xmap[xsum + j] = (uint32_t)npred;
+        xsum += nb;
+        ++npred;
+      }
       p.h_large[il++] = d;
       base += ts.len;
       groups += pred ? sample_groups(ts.len, ts.k) : 0;
@@ -266,6 +328,39 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.h_sbases[il] = groups;
   p.h_cbases[il] = runs;
   p.h_mbases[il] = msum;
+  p.h_xbases[npred] = xsum;
+  p.n_pred = npred;
+  p.total_split = xsum;
+  // Launch shapes of the streaming kernels and their block -> task maps.
+  p.grid_h = p.grid_d = p.grid_p = p.grid_s = 0;
+  p.chunk_h = nl ? stream_chunk(tot, f, o, &p.grid_h) : 0;
+  p.chunk_d = (tot > tot_pred) ? stream_chunk(tot - tot_pred, f, o, &p.grid_d) : 0;
+  p.chunk_p = (tot_pred > 0) ? stream_chunk(tot_pred, f, o, &p.grid_p) : 0;
+  p.grid_s = groups > 0 ? (int)((groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK) : 0;
+  p.off_h = 0;
+  p.off_d = p.off_h + p.grid_h;
+  p.off_p = p.off_d + p.grid_d;
+  p.off_s = p.off_p + p.grid_p;
+  p.n_bmap = p.off_s + p.grid_s;
+  if (!p.h_bmap || (size_t)p.n_bmap > p.cap_bmap) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = 64;
+    while (c < (size_t)p.n_bmap) c *= 2;
+    if (p.h_bmap) RADIK_HIP(hipHostFree(p.h_bmap));
+    if (p.d_bmap) RADIK_HIP(hipFree(p.d_bmap));
+    p.h_bmap = nullptr;
+    p.d_bmap = nullptr;
+    if (hipHostMalloc((void**)&p.h_bmap, c * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&p.d_bmap, c * sizeof(uint32_t)) != hipSuccess) {
+      p.cap_bmap = 0;
+      return set_error(6, "allocation of a %zu-entry block map failed", c);
+    }
+    p.cap_bmap = c;
+  }
+  fill_block_map(p.h_bmap + p.off_h, p.grid_h, 0, p.chunk_h, p.h_bases, nl);
+  fill_block_map(p.h_bmap + p.off_d, p.grid_d, tot_pred, p.chunk_d, p.h_bases, nl);
+  fill_block_map(p.h_bmap + p.off_p, p.grid_p, 0, p.chunk_p, p.h_bases, nl);
+  fill_block_map(p.h_bmap + p.off_s, p.grid_s, 0, SAMPLE_GROUPS_PER_BLOCK, p.h_sbases, nl);
   p.total_runs = runs;
   p.total_m = msum;
   p.n_large = nl;
@@ -284,6 +379,11 @@ int upload(Pipeline& p, hipStream_t s) {
     RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_cbases, p.h_cbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_mbases, p.h_mbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    RADIK_HIP(hipMemcpyAsync(p.d_xbases, p.h_xbases, (p.n_pred + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    if (p.total_split)
+      RADIK_HIP(hipMemcpyAsync(p.d_xmap, p.h_xmap, p.total_split * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (p.n_bmap)
+      RADIK_HIP(hipMemcpyAsync(p.d_bmap, p.h_bmap, p.n_bmap * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   }
   if (p.n_small)
     RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
@@ -295,16 +395,16 @@ int upload(Pipeline& p, hipStream_t s) {
 // Kernel sequence
 // ---------------------------------------------------------------------------
 template <int UNR>
-static void launch_stream_kernels_hist(const Pipeline& p, int64_t chunk, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_hist<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     p.total, chunk, p.d_hist, p.d_ctl, p.d_cand);
+static void launch_stream_kernels_hist(const Pipeline& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_hist<UNR>, dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
+                     p.total, p.chunk_h, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h);
 }
 
 template <int UNR>
 static void launch_stream_kernels_compact(const Pipeline& p, int64_t g_begin, int64_t g_end, int64_t chunk,
-                                          int grid, hipStream_t s) {
+                                          int grid, const uint32_t* bmap, hipStream_t s) {
   hipLaunchKernelGGL(k_compact<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     g_begin, g_end, chunk, p.d_ctl, p.d_cand);
+                     g_begin, g_end, chunk, p.d_ctl, p.d_cand, bmap);
 }
 
 // Equal chunks of `total` elements, about blocks_per_cu blocks per CU.
@@ -328,13 +428,21 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
   };
   int rc;
   if ((rc = mark())) return rc;  // start
+  // Short tasks are independent of the long ones: outside the phase timing
+  // they run on the side stream, overlapping the streaming passes.
+  const bool fork = !timed && p.n_small && p.n_large && p.side;
+  if (fork) {
+    RADIK_HIP(hipEventRecord(p.ev_fork, s));
+    RADIK_HIP(hipStreamWaitEvent(p.side, p.ev_fork, 0));
+    const uint32_t scap = eff_small_cap(f, o);
+    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, p.side, p.d_small, scap);
+    RADIK_HIP(hipGetLastError());
+    RADIK_HIP(hipEventRecord(p.ev_join, p.side));
+  }
   if (p.n_large) {
-    int grid = 0;
-    const int64_t chunk = stream_chunk(p.total, f, o, &grid);
     if (p.total_groups > 0) {
-      const int sgrid = (int)((p.total_groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK);
-      hipLaunchKernelGGL(k_sample, dim3(sgrid), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
-                         p.total_groups, p.d_shist);
+      hipLaunchKernelGGL(k_sample, dim3(p.grid_s), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
+                         p.total_groups, p.d_shist, p.d_bmap + p.off_s);
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
@@ -343,13 +451,28 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
     switch (o.unr) {
-      case 1: launch_stream_kernels_hist<1>(p, chunk, grid, s); break;
-      case 4: launch_stream_kernels_hist<4>(p, chunk, grid, s); break;
-      default: launch_stream_kernels_hist<2>(p, chunk, grid, s); break;
+      case 1: launch_stream_kernels_hist<1>(p, s); break;
+      case 4: launch_stream_kernels_hist<4>(p, s); break;
+      default: launch_stream_kernels_hist<2>(p, s); break;
     }
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl);
+    // Register sort blocking of k_sort: the launch's largest m, and room for
+    // the tasks it finishes from their compacted winners and candidates
+    // (not with the multi-block sort, which reads staged output slots).
+    const bool multi_sort = p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS;
+    uint32_t e = 1;
+    while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;
+    // Finished tasks keep nwin + ncand <= fin_cap <= FIN_CAP entries and
+    // their m results in LDS: FIN_CAP + min(max_m, FIN_CAP) entries.
+    uint32_t fin_cap = 0;
+    size_t sort_lds = 0;
+    if (!multi_sort && o.fin_cap > 0) {
+      fin_cap = std::min<uint32_t>(o.fin_cap, FIN_CAP);
+      sort_lds = ((size_t)FIN_CAP + std::min<uint32_t>(p.max_m, FIN_CAP)) * 8;
+    }
+    hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl,
+                       fin_cap);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
     // Pass B: the tasks without a prediction (all of them re-read), then the
@@ -358,22 +481,28 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
       const int64_t gb = part == 0 ? p.total_pred : 0;
       const int64_t ge = part == 0 ? p.total : p.total_pred;
       if (ge <= gb) continue;
-      int g2 = 0;
-      const int64_t c2 = stream_chunk(ge - gb, f, o, &g2);
+      const int64_t c2 = part == 0 ? p.chunk_d : p.chunk_p;
+      const int g2 = part == 0 ? p.grid_d : p.grid_p;
+      const uint32_t* bm = p.d_bmap + (part == 0 ? p.off_d : p.off_p);
       switch (o.unr) {
-        case 1: launch_stream_kernels_compact<1>(p, gb, ge, c2, g2, s); break;
-        case 4: launch_stream_kernels_compact<4>(p, gb, ge, c2, g2, s); break;
-        default: launch_stream_kernels_compact<2>(p, gb, ge, c2, g2, s); break;
+        case 1: launch_stream_kernels_compact<1>(p, gb, ge, c2, g2, bm, s); break;
+        case 4: launch_stream_kernels_compact<4>(p, gb, ge, c2, g2, bm, s); break;
+        default: launch_stream_kernels_compact<2>(p, gb, ge, c2, g2, bm, s); break;
       }
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
+    if (p.n_pred > 0) {
+      hipLaunchKernelGGL(k_split, dim3((unsigned)p.total_split), dim3(BLOCK), 0, s, p.d_large, p.d_xbases,
+                         p.d_xmap, p.d_ctl, p.d_cand);
+      RADIK_HIP(hipGetLastError());
+    }
     const uint32_t rcap = eff_refine_cap(f, o);
     hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)rcap * 8, s, p.d_large,
                        p.d_ctl, p.d_cand, rcap);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    if (p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS) {
+    if (multi_sort) {
       // several blocks per task: sorted runs, then ranks across runs
       hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)p.total_runs), dim3(BLOCK), 0, s, p.d_large, p.d_ctl,
                          p.d_cbases, (int)p.n_large, p.d_cand);
@@ -382,10 +511,7 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
       hipLaunchKernelGGL(k_sort_rank, dim3(rg), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_mbases,
                          (int)p.n_large, p.total_m, (const uint64_t*)p.d_cand);
     } else {
-      // results per thread of the register sort: the launch's largest m
-      uint32_t e = 1;
-      while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;
-      const size_t lds = (size_t)BLOCK * e * 8;
+      const size_t lds = std::max<size_t>((size_t)BLOCK * e * 8, sort_lds);
       const dim3 g((unsigned)p.n_large), b(BLOCK);
       switch (e) {
         case 1: hipLaunchKernelGGL(k_sort<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
@@ -403,7 +529,9 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
       if ((rc = mark())) return rc;
   }
   if ((rc = mark())) return rc;  // end of sort
-  if (p.n_small) {
+  if (fork) {
+    RADIK_HIP(hipStreamWaitEvent(s, p.ev_join, 0));
+  } else if (p.n_small) {
     const uint32_t scap = eff_small_cap(f, o);
     hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, s, p.d_small, scap);
     RADIK_HIP(hipGetLastError());
@@ -436,19 +564,16 @@ int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
 
 int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s) {
   if (!p.n_large) return 0;
-  LaunchOpts o;
-  int grid = 0;
-  const int64_t chunk = stream_chunk(p.total, f, o, &grid);
   // Control blocks without a prediction: pass A only counts.
   // staged without prediction: k_predict only initialises the control blocks
   hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
                      p.d_ctl);
   RADIK_HIP(hipGetLastError());
-  launch_stream_kernels_hist<2>(p, chunk, grid, s);
+  launch_stream_kernels_hist<2>(p, s);
   RADIK_HIP(hipGetLastError());
   RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),
                            hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl);
+  hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl, 0u);
   RADIK_HIP(hipGetLastError());
   RADIK_HIP(hipMemcpy2DAsync(sel_out, 4 * sizeof(uint32_t), p.d_ctl, sizeof(TaskCtl), 4 * sizeof(uint32_t),
                              (size_t)p.n_large, hipMemcpyDeviceToDevice, s));
