@@ -31,11 +31,13 @@ struct TaskDesc {
                         // also the task's region of the candidate buffer
   int32_t k;            // requested k (>= 0)
   uint32_t flags;       // F_* bits
-  int64_t reserved;
+  int32_t nblk;         // pass-A blocks whose chunk overlaps the task
+  int32_t spare;
 };
 static_assert(sizeof(TaskDesc) == 64, "TaskDesc layout");
 
-// Per-task state between the passes, written by k_select.
+// Per-task state between the passes: k_predict initialises it, the
+// first-digit selection (end of pass A) fills it.
 struct TaskCtl {
   uint32_t b_win;     // first-digit values < b_win are winners
   uint32_t b_cand;    // first-digit value of the candidates, NO_BIN if none
@@ -47,10 +49,9 @@ struct TaskCtl {
   uint32_t cand_ctr;  // entries in the task's spill region
   uint32_t b_pred;    // predicted bound: pass A spills first digits <= b_pred (NO_BIN: no spill)
   uint32_t spill_all; // 1: the spill holds winners and candidates; 0: pass B re-reads the task
-  uint32_t fin;       // 1: k_split compacts winners and candidates, k_sort finishes the task
-  uint32_t cc_ctr;    // candidates compacted behind the spill by k_split
-  uint64_t c_and;     // bitwise AND of the spilled candidates (k_split)
-  uint64_t c_or;      // bitwise OR of the spilled candidates (k_split)
+  uint32_t fin;       // 1: winners + candidates fit k_finish's LDS (nwin + ncand <= its capacity)
+  uint32_t done;      // pass-A blocks of the task that finished (the last one selects)
+  uint32_t pad[4];
 };
 static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 

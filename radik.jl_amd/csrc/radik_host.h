@@ -35,6 +35,12 @@ struct DevFacts {
 };
 int dev_facts(DevFacts** out);
 
+// Tasks up to BIG_LEN elements without a usable spill are re-read by one
+// k_finish block; longer ones by pass B over several blocks (k_compact).
+constexpr int64_t BIG_LEN = (int64_t)1 << 20;
+// Static LDS of k_finish beside its dynamic arrays (upper bound, bytes).
+constexpr size_t FINISH_STATIC_LDS = 2048;
+
 // Launch-level options.
 struct LaunchOpts {
   int unr = 2;              // float4 loads in flight per thread (vload bytes / 16)
@@ -42,7 +48,6 @@ struct LaunchOpts {
   uint32_t small_cap = 0;   // longest task for k_small (0: device default)
   uint32_t refine_cap = 0;  // candidates held in LDS by k_refine (0: device default)
   bool predict = true;      // sample long tasks and spill in pass A
-  uint32_t fin_cap = 2048;  // winners + candidates sorted directly by k_sort (0: never)
 };
 inline uint32_t eff_small_cap(const DevFacts& f, const LaunchOpts& o) {
   return o.small_cap ? (o.small_cap < f.small_cap_max ? o.small_cap : f.small_cap_max) : f.small_cap;
@@ -60,8 +65,6 @@ struct Pipeline {
   int64_t* h_sbases = nullptr;  // sample-group prefix sums of the long tasks
   int64_t* h_cbases = nullptr;  // sorted-run prefix sums (multi-block sort)
   int64_t* h_mbases = nullptr;  // result-count prefix sums (multi-block sort)
-  int64_t* h_xbases = nullptr;  // k_split block prefix sums of the predicted tasks
-  uint32_t* h_xmap = nullptr;   // task of each k_split block
   uint32_t* h_bmap = nullptr;   // first task of each block of the streaming launches
   TaskDesc* h_small = nullptr;
   // device
@@ -70,24 +73,22 @@ struct Pipeline {
   int64_t* d_sbases = nullptr;
   int64_t* d_cbases = nullptr;
   int64_t* d_mbases = nullptr;
-  int64_t* d_xbases = nullptr;
-  uint32_t* d_xmap = nullptr;
   uint32_t* d_bmap = nullptr;
   TaskDesc* d_small = nullptr;
   TaskCtl* d_ctl = nullptr;
   uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
   uint32_t* d_shist = nullptr;  // sample histograms, zero between launches
   uint64_t* d_cand = nullptr;   // spill regions
-  size_t cap_large = 0, cap_small = 0, cap_hist = 0, cap_cand = 0, cap_xmap = 0, cap_bmap = 0;
-  // Streaming launch shapes fixed at staging (pass A, pass B over the
-  // unpredicted / predicted tasks, sampling): chunk, grid, offset in bmap.
-  int64_t chunk_h = 0, chunk_d = 0, chunk_p = 0;
-  int grid_h = 0, grid_d = 0, grid_p = 0, grid_s = 0;
-  int64_t off_h = 0, off_d = 0, off_p = 0, off_s = 0, n_bmap = 0;
+  size_t cap_large = 0, cap_small = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0;
+  // Streaming launch shapes fixed at staging (passes A and B share one
+  // chunking; sampling has its own): chunk, grid, offset of the block map.
+  int64_t chunk_h = 0;
+  int grid_h = 0, grid_s = 0;
+  int64_t off_h = 0, off_s = 0, n_bmap = 0;
   // current content
   int64_t n_large = 0, n_small = 0, total = 0, total_groups = 0;
   int64_t total_pred = 0;  // elements of the predicted tasks, which come first
-  int64_t n_pred = 0, total_split = 0;  // predicted tasks, k_split blocks
+  bool has_big = false;    // some long task exceeds BIG_LEN (pass B over several blocks)
   uint32_t max_m = 0;  // largest min(k, len) over the long tasks
   int64_t total_runs = 0, total_m = 0;  // multi-block sort sizes
   hipEvent_t upload_done = nullptr;  // pinned buffers reusable after this
@@ -96,7 +97,7 @@ struct Pipeline {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   ~Pipeline();
-  int reserve(int64_t n_large, int64_t n_small, int64_t total_large, int64_t split_blocks, hipStream_t s);
+  int reserve(int64_t n_large, int64_t n_small, int64_t total_large, hipStream_t s);
 };
 
 // Splits tasks into short ones (k_small) and long ones (streaming passes),

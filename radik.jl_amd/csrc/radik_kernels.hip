@@ -287,6 +287,104 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// First-digit selection of one task from its complete 4096-bin histogram (c:
+// this thread's 16 consecutive bins), called by the whole block.  Decides the
+// winner bound, the candidate bin and the residual k, whether the spill of
+// pass A holds every key the task needs (first digit <= the selected bin) or
+// the task is re-read, and whether its winners and candidates fit k_finish's
+// LDS (fin_cap entries).  b_pred / spilled: the prediction and the spill
+// count of pass A.  Thread 0 writes the control block.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void select_first(const TaskDesc& d, const uint32_t (&c)[16], uint32_t b_pred,
+                                             uint32_t spilled, uint32_t fin_cap, TaskCtl* out,
+                                             uint32_t* wsum, uint32_t* res) {
+  const int64_t len = d.len;
+  int64_t m = d.k < len ? (int64_t)d.k : len;
+  if (threadIdx.x == 0) res[0] = NO_BIN;  // published by the barriers inside the scan
+  if (m > 0 && m < len) block_find_bin<16>(c, (uint32_t)m, wsum, res);
+  if (threadIdx.x == 0) {
+    TaskCtl o = {};
+    o.b_cand = NO_BIN;
+    o.krem = 0;
+    o.b_pred = b_pred;
+    if (m <= 0) {
+      o.b_win = 0;
+    } else if (m >= len) {
+      o.b_win = HIST_BINS;
+    } else if (res[0] >= (uint32_t)HIST_BINS || res[1] >= (uint32_t)m || res[2] == 0 ||
+               (uint32_t)m - res[1] > res[2]) {
+      // The histogram does not hold the task's elements (cannot happen when
+      // the buffer was zero): emit nothing rather than overrun the outputs.
+      o.b_win = 0;
+      m = 0;
+    } else {
+      const uint32_t b = res[0], before = res[1], cnt = res[2];
+      const uint32_t krem = (uint32_t)m - before;
+      if (krem == cnt) {
+        o.b_win = b + 1;  // the whole bin is in: no candidates
+      } else {
+        o.b_win = b;
+        o.b_cand = b;
+        o.krem = krem;
+        o.ncand = cnt;
+      }
+    }
+    o.m = (uint32_t)(m > 0 ? m : 0);
+    o.nwin = o.m - o.krem;
+    o.win_ctr = 0;
+    o.done = 1;
+    // Highest first digit the task needs: the candidate bin, or the last
+    // winner bin when the threshold bin is taken whole.
+    const uint32_t need = (o.b_cand != NO_BIN) ? o.b_cand : (o.b_win - 1);
+    const uint64_t both = (uint64_t)o.nwin + o.ncand;
+    o.fin = (o.m > 0 && both <= fin_cap) ? 1u : 0u;
+    if (o.m == 0) {
+      o.spill_all = 1;
+      o.cand_ctr = 0;
+    } else if (b_pred == NO_BIN || need > b_pred || spilled > (uint32_t)len) {
+      o.spill_all = 0;
+      o.cand_ctr = 0;
+    } else {
+      o.spill_all = 1;
+      o.cand_ctr = spilled;
+    }
+    *out = o;
+  }
+}
+
+// Reads this thread's 16 bins of a task's global histogram and clears them
+// (the buffer is zero again for the next launch).
+__device__ __forceinline__ void take_hist(uint32_t* __restrict__ h, uint32_t (&c)[16]) {
+  uint4* __restrict__ h4 = reinterpret_cast<uint4*>(h) + threadIdx.x * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint4 v = h4[q];
+    c[4 * q + 0] = v.x;
+    c[4 * q + 1] = v.y;
+    c[4 * q + 2] = v.z;
+    c[4 * q + 3] = v.w;
+    h4[q] = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
+// Standalone selection, one block per task, of the tasks pass A did not
+// select (those spread over several of its blocks): reads and clears the
+// task's histogram.
+__global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ tasks,
+                                                  uint32_t* __restrict__ hist,
+                                                  TaskCtl* __restrict__ ctl, uint32_t fin_cap) {
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t res[3];
+  const int t = blockIdx.x;
+  const TaskCtl prev = ctl[t];
+  if (prev.done) return;  // selected at the end of pass A
+  const TaskDesc d = tasks[t];
+  uint32_t c[16];
+  take_hist(hist + (size_t)t * HIST_BINS, c);
+  select_first(d, c, prev.b_pred, prev.cand_ctr, fin_cap, &ctl[t], wsum, res);
+}
+
+// ---------------------------------------------------------------------------
 // Pass A: exact first-digit histogram of every task; for predicted tasks the
 // same read also spills every key with first digit <= b_pred (composite
 // rank << 32 | position) into the task's spill region.
@@ -307,17 +405,30 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
   if (SPILL) stage_flush(st, ctr, write);
 }
 
-template <int UNR>
+// Relaxed device-scope load: reads where device-scope atomics are performed,
+// so it sees every atomic update that completed before it.
+__device__ __forceinline__ uint32_t load_dev(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+
+// SELECT: a task inside one block's chunk (nblk == 1) is selected from that
+// block's LDS histogram right after its segment, without a global histogram;
+// the others (and every task without SELECT) add their counts to the task's
+// global histogram, selected after the pass.
+template <int UNR, bool SELECT>
 __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tasks,
                                                 const int64_t* __restrict__ bases, int ntasks,
                                                 int64_t total, int64_t chunk,
                                                 uint32_t* __restrict__ hist,
                                                 TaskCtl* __restrict__ ctl,
                                                 uint64_t* __restrict__ cand,
-                                                const uint32_t* __restrict__ bmap) {
+                                                const uint32_t* __restrict__ bmap, uint32_t fin_cap) {
   __shared__ uint32_t lh[HIST_BINS];
   __shared__ uint64_t stage_buf[BLOCK / 64][STAGE];
   __shared__ uint32_t stage_n[BLOCK / 64];
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t res[3];
   for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
   if (threadIdx.x < BLOCK / 64) stage_n[threadIdx.x] = 0;
   const WaveStage st{stage_buf[threadIdx.x >> 6], &stage_n[threadIdx.x >> 6]};
@@ -346,7 +457,20 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
       else pass_a_segment<UNR, false, true>(d, s, e, sc, xm, smax, lh, ctr, spill, st);
     }
     __syncthreads();
-    flush_hist(lh, hist + (size_t)t * HIST_BINS);
+    if (!SELECT) {
+      flush_hist(lh, hist + (size_t)t * HIST_BINS);
+    } else if (d.nblk <= 1) {
+      // the whole task was in this chunk: select from the LDS histogram
+      uint32_t c[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        c[j] = lh[threadIdx.x * 16 + j];
+        lh[threadIdx.x * 16 + j] = 0;
+      }
+      select_first(d, c, bp, load_dev(ctr), fin_cap, &ctl[t], wsum, res);
+    } else {
+      flush_hist(lh, hist + (size_t)t * HIST_BINS);  // selected after the pass (k_select or k_finish)
+    }
     if (se >= g1) break;
     g0 = se;
     ++t;
@@ -356,95 +480,11 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
 }
 
 // ---------------------------------------------------------------------------
-// First-digit selection, one block per task.  Reads and clears the task's
-// histogram so the buffer is zero again for the next launch.  Decides whether
-// the spill of pass A already holds every key the task needs (first digit <=
-// the selected bin) or pass B must re-read the task.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ tasks,
-                                                  uint32_t* __restrict__ hist,
-                                                  TaskCtl* __restrict__ ctl, uint32_t fin_cap) {
-  __shared__ uint32_t wsum[BLOCK / 64];
-  __shared__ uint32_t res[3];
-  const int t = blockIdx.x;
-  const TaskDesc d = tasks[t];
-  uint4* __restrict__ h4 = reinterpret_cast<uint4*>(hist + (size_t)t * HIST_BINS) + threadIdx.x * 4;
-  uint32_t c[16];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint4 v = h4[q];
-    c[4 * q + 0] = v.x;
-    c[4 * q + 1] = v.y;
-    c[4 * q + 2] = v.z;
-    c[4 * q + 3] = v.w;
-    h4[q] = make_uint4(0u, 0u, 0u, 0u);
-  }
-  const int64_t len = d.len;
-  int64_t m = d.k < len ? (int64_t)d.k : len;
-  if (threadIdx.x == 0) res[0] = NO_BIN;  // published by the barriers inside the scan
-  if (m > 0 && m < len) block_find_bin<16>(c, (uint32_t)m, wsum, res);
-  if (threadIdx.x == 0) {
-    const TaskCtl prev = ctl[t];
-    TaskCtl o = {};
-    o.b_cand = NO_BIN;
-    o.krem = 0;
-    o.b_pred = prev.b_pred;
-    if (m <= 0) {
-      o.b_win = 0;
-    } else if (m >= len) {
-      o.b_win = HIST_BINS;
-    } else if (res[0] >= (uint32_t)HIST_BINS || res[1] >= (uint32_t)m || res[2] == 0 ||
-               (uint32_t)m - res[1] > res[2]) {
-      // The histogram does not hold the task's elements (cannot happen when
-      // the buffer was zero): emit nothing rather than overrun the outputs.
-      o.b_win = 0;
-      m = 0;
-    } else {
-      const uint32_t b = res[0], before = res[1], cnt = res[2];
-      const uint32_t krem = (uint32_t)m - before;
-      if (krem == cnt) {
-        o.b_win = b + 1;  // the whole bin is in: no candidates
-      } else {
-        o.b_win = b;
-        o.b_cand = b;
-        o.krem = krem;
-        o.ncand = cnt;
-      }
-    }
-    o.m = (uint32_t)(m > 0 ? m : 0);
-    o.nwin = o.m - o.krem;
-    o.win_ctr = 0;
-    // Highest first digit the task needs: the candidate bin, or the last
-    // winner bin when the threshold bin is taken whole.
-    const uint32_t need = (o.b_cand != NO_BIN) ? o.b_cand : (o.b_win - 1);
-    // Few enough winners + candidates: k_sort ranks both and finishes the
-    // task from the spill region (behind the spill, or at its start after
-    // pass B).
-    const uint64_t both = (uint64_t)o.nwin + o.ncand;
-    o.fin = 0;
-    if (o.m == 0) {
-      o.spill_all = 1;
-      o.cand_ctr = 0;
-    } else if (prev.b_pred == NO_BIN || need > prev.b_pred || prev.cand_ctr > (uint32_t)len) {
-      o.spill_all = 0;
-      o.cand_ctr = 0;
-      o.fin = (both <= fin_cap && both <= (uint64_t)len) ? 1u : 0u;
-    } else {
-      o.spill_all = 1;
-      o.cand_ctr = prev.cand_ctr;
-      o.fin = (both <= fin_cap && (uint64_t)o.cand_ctr + both <= (uint64_t)len) ? 1u : 0u;
-    }
-    o.cc_ctr = 0;
-    o.c_and = ~0ull;
-    o.c_or = 0ull;
-    ctl[t] = o;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Pass B (only tasks whose prediction was missing or too tight): winners
+// Pass B over several blocks, for tasks longer than min_len whose prediction
+// was missing or too tight (k_finish re-reads shorter ones itself): winners
 // (digit < bin) go to the output slots [0, nwin) as staging (value,
-// position); candidates (digit == bin) to the task's spill region.
+// position), candidates (digit == bin) to the task's spill region; with fin,
+// both go to the spill region, winners first, as selection composites.
 // ---------------------------------------------------------------------------
 template <int UNR, bool SCALE>
 __device__ __forceinline__ void pass_b_segment(const TaskDesc& d, int64_t s, int64_t e, float sc, uint32_t xm,
@@ -487,7 +527,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
                                                    int64_t g_begin, int64_t total, int64_t chunk,
                                                    TaskCtl* __restrict__ ctl,
                                                    uint64_t* __restrict__ cand,
-                                                   const uint32_t* __restrict__ bmap) {
+                                                   const uint32_t* __restrict__ bmap, int64_t min_len) {
   __shared__ uint64_t stage_w[BLOCK / 64][STAGE];
   __shared__ uint64_t stage_c[BLOCK / 64][STAGE];
   __shared__ uint32_t stage_n[2][BLOCK / 64];
@@ -504,7 +544,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
     const int64_t tb = bases[t];
     const int64_t se = min(g1, tb + d.len);
     const TaskCtl c = ctl[t];
-    if (!c.spill_all && c.m > 0) {
+    if (!c.spill_all && c.m > 0 && d.len > min_len) {
       const float sc = task_scaler(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
       uint32_t* wctr = &ctl[t].win_ctr;
@@ -535,9 +575,9 @@ struct GlobalSrc {
 };
 
 // Global candidates: 16-byte loads (two entries), GLOBAL_LOADS in flight per
-// thread, so one block keeps 64 KiB of requests outstanding.  A head entry
+// thread, so one block keeps 32 KiB of requests outstanding.  A head entry
 // before 16-byte alignment and an odd tail entry go to thread 0.
-constexpr int GLOBAL_LOADS = 16;
+constexpr int GLOBAL_LOADS = 8;
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
 template <typename F>
@@ -666,199 +706,6 @@ __device__ __forceinline__ float spilled_value(const TaskDesc& d, uint64_t x, ui
   if (d.flags & F_WITHSCALE) return d.in[pos];
   const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
   return fkey_inv((uint32_t)(x >> 32) ^ xm);
-}
-
-template <typename Src>
-__device__ __forceinline__ void refine_emit(const TaskDesc& d, const TaskCtl& c, Src src, uint32_t n,
-                                            const unsigned long long* vb, uint32_t* lh, uint32_t* wsum,
-                                            uint32_t* res, uint32_t* ectr) {
-  uint64_t P = c.b_cand;
-  const uint64_t vary = (uint64_t)(vb[0] ^ vb[1]);
-  const int s = select_digits(src, n, 52, P, c.krem, vary, (uint64_t)vb[0], lh, wsum, res);
-  const uint32_t lim = c.m - c.nwin;
-  const uint64_t bc = c.b_cand;
-  for_each_entry(src, n, [&](uint64_t x) {
-    const bool take = (x >> 52) == bc && (x >> s) <= P;
-    const uint64_t tm = ballot(take);
-    if (tm) {
-      const uint32_t slot = wave_append(take, tm, ectr);
-      if (take && slot < lim) {
-        uint32_t pos;
-        const float v = spilled_value(d, x, pos);
-        put_result(d, c.nwin + slot, v, pos);
-      }
-    }
-  });
-}
-
-// Candidates of a spilled task are compacted behind its spill, at [n, n +
-// ncand) of the spill region (n = spilled entries), when that stays inside
-// the region; otherwise k_refine reads them in place among the spill.
-__device__ __forceinline__ bool cands_compacted(const TaskCtl& c, int64_t len) {
-  return c.spill_all && c.krem > 0 && (uint64_t)c.cand_ctr + c.ncand <= (uint64_t)len;
-}
-
-// ---------------------------------------------------------------------------
-// Split of the pass-A spills, several blocks per predicted task (xmap: task
-// of each block, xbases: prefix sums of the blocks per task; the predicted
-// tasks come first).  Each
-// block takes an equal share of its task's spill: winners (first digit <
-// b_win) go to the output slots [0, nwin), candidates (digit == b_cand) are
-// compacted behind the spill, and the candidates' bitwise AND / OR are
-// merged into the control block for k_refine's digit skipping.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ tasks,
-                                                 const int64_t* __restrict__ xbases,
-                                                 const uint32_t* __restrict__ xmap,
-                                                 TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand) {
-  constexpr int U = LOADS_IN_FLIGHT;
-  __shared__ unsigned long long vb[2];
-  __shared__ uint32_t wsum[BLOCK / 64];
-  __shared__ uint32_t slot0[2];
-  const uint32_t t = xmap[blockIdx.x];
-  const TaskCtl c = ctl[t];
-  if (!c.spill_all || c.m == 0) return;
-  const TaskDesc d = tasks[t];
-  uint32_t n = c.cand_ctr;
-  if (n > (uint32_t)d.len) n = (uint32_t)d.len;
-  const uint32_t nb = (uint32_t)(xbases[t + 1] - xbases[t]);
-  const uint32_t j = (uint32_t)(blockIdx.x - xbases[t]);
-  const uint32_t per = (n + nb - 1) / nb;
-  const uint32_t s = min(n, j * per), e = min(n, s + per);
-  if (s >= e) return;
-  // fin: winners and candidates both go behind the spill (winners first);
-  // else winners to the output slots and candidates behind the spill if they fit.
-  const bool fin = c.fin != 0;
-  const bool compact = fin || cands_compacted(c, d.len);
-  uint64_t* __restrict__ spill = cand + d.base;
-  uint64_t* __restrict__ cdst = spill + n + (fin ? c.nwin : 0u);
-  if (threadIdx.x == 0) {
-    vb[0] = ~0ull;
-    vb[1] = 0ull;
-  }
-  uint64_t a = ~0ull, o = 0ull;
-  const uint32_t bw = c.b_win, bc = c.b_cand;
-  // Tiles of BLOCK * U entries: one block scan and one pair of global atomics
-  // place all of a tile's winners and candidates.
-  for (uint32_t tile = s; tile < e; tile += (uint32_t)BLOCK * U) {
-    uint64_t xs[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t i = tile + (uint32_t)u * BLOCK + threadIdx.x;
-      xs[u] = (i < e) ? spill[i] : 0ull;
-    }
-    uint32_t wmask = 0, cmask = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (tile + (uint32_t)u * BLOCK + threadIdx.x >= e) continue;
-      const uint32_t dg = (uint32_t)(xs[u] >> 52);
-      if (dg < bw) {
-        wmask |= 1u << u;
-      } else if (dg == bc) {
-        cmask |= 1u << u;
-        a &= xs[u];
-        o |= xs[u];
-      }
-    }
-    const uint32_t mine = (uint32_t)__builtin_popcount(wmask) | ((compact ? (uint32_t)__builtin_popcount(cmask) : 0u) << 16);
-    const uint32_t incl = block_incl_scan(mine, wsum);
-    if (threadIdx.x == BLOCK - 1) {
-      const uint32_t tw = incl & 0xFFFFu, tc = incl >> 16;
-      slot0[0] = tw ? atomicAdd(&ctl[t].win_ctr, tw) : 0u;
-      slot0[1] = tc ? atomicAdd(&ctl[t].cc_ctr, tc) : 0u;
-    }
-    __syncthreads();
-    const uint32_t excl = incl - mine;
-    uint32_t ws = slot0[0] + (excl & 0xFFFFu), cs = slot0[1] + (excl >> 16);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (wmask & (1u << u)) {
-        if (ws < c.nwin) {
-          if (fin) {
-            spill[n + ws] = xs[u];
-          } else {
-            uint32_t pos;
-            const float v = spilled_value(d, xs[u], pos);
-            put_result(d, ws, v, pos);
-          }
-        }
-        ++ws;
-      } else if (compact && (cmask & (1u << u))) {
-        if (cs < c.ncand) cdst[cs] = xs[u];
-        ++cs;
-      }
-    }
-  }
-  if (c.krem == 0 || fin) return;
-  block_and_or(a, o, vb);
-  // (any candidate leaves the AND below all-ones or the OR above zero)
-  if (threadIdx.x == 0 && (vb[0] != ~0ull || vb[1] != 0ull)) {
-    atomicAnd((unsigned long long*)&ctl[t].c_and, vb[0]);
-    atomicOr((unsigned long long*)&ctl[t].c_or, vb[1]);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Refinement, one block per task with candidates left: resolves the
-// threshold bin's candidates digit by digit (in LDS when they fit) and emits
-// the chosen ones behind the winners.  The candidates are the compacted ones
-// of k_split, the spill itself when they could not be compacted, or pass B's
-// candidate list.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_refine(const TaskDesc* __restrict__ tasks,
-                                                  const TaskCtl* __restrict__ ctl,
-                                                  const uint64_t* __restrict__ cand,
-                                                  uint32_t lds_cap) {
-  extern __shared__ uint64_t cbuf[];
-  __shared__ uint32_t lh[HIST_BINS];
-  __shared__ uint32_t wsum[BLOCK / 64];
-  __shared__ uint32_t res[3];
-  __shared__ uint32_t ectr;
-  __shared__ unsigned long long vb[2];
-  const int t = blockIdx.x;
-  const TaskCtl c = ctl[t];
-  if (c.m == 0 || c.krem == 0 || c.fin) return;
-  const TaskDesc d = tasks[t];
-  uint32_t n = c.cand_ctr;
-  if (n > (uint32_t)d.len) n = (uint32_t)d.len;
-  const uint64_t* __restrict__ src = cand + d.base;
-  if (cands_compacted(c, d.len)) {
-    src += n;
-    n = min(c.cc_ctr, c.ncand);
-  }
-  // Spilled tasks bring the AND / OR from k_split; pass B's lists get them here.
-  const bool have_vb = c.spill_all != 0;
-  if (threadIdx.x == 0) {
-    ectr = 0;
-    vb[0] = have_vb ? (unsigned long long)c.c_and : ~0ull;
-    vb[1] = have_vb ? (unsigned long long)c.c_or : 0ull;
-  }
-  __syncthreads();
-  uint64_t a = ~0ull, o = 0ull;
-  const bool in_lds = n <= lds_cap;
-  if (in_lds || !have_vb) {
-    for (uint32_t b = threadIdx.x; b < n; b += BLOCK * LOADS_IN_FLIGHT) {
-      uint64_t xs[LOADS_IN_FLIGHT];
-#pragma unroll
-      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
-        const uint32_t i = b + (uint32_t)u * BLOCK;
-        xs[u] = (i < n) ? src[i] : 0ull;
-      }
-#pragma unroll
-      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
-        const uint32_t i = b + (uint32_t)u * BLOCK;
-        if (i < n) {
-          if (in_lds) cbuf[i] = xs[u];
-          a &= xs[u];
-          o |= xs[u];
-        }
-      }
-    }
-  }
-  if (!have_vb) block_and_or(a, o, vb);
-  else __syncthreads();
-  if (in_lds) refine_emit(d, c, LdsSrc{cbuf}, n, vb, lh, wsum, res, &ectr);
-  else refine_emit(d, c, GlobalSrc{src}, n, vb, lh, wsum, res, &ectr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1086,70 +933,6 @@ __device__ __forceinline__ void sort_selected(const TaskDesc& d, const uint64_t*
   sort_selected_regs<EMAX>(d, sel, m, xs);
 }
 
-// LDS of a finishing block beside the dynamic arrays.
-struct FinLds {
-  uint32_t lh[256];
-  uint32_t wsum[BLOCK / 64];
-  uint32_t res[3];
-  uint32_t ectr;
-  unsigned long long vb[2];
-};
-
-// Finishes a task whose nwin winners and ncand candidates lie compacted at
-// src (winners first): loads them to v (LDS), resolves the krem best
-// candidates with narrow digits, gathers winners and chosen candidates in
-// sel and sorts those by output order.  fin_cap entries of v; EMAX covers
-// fin_cap.
-template <int EMAX>
-__device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c, const uint64_t* __restrict__ src,
-                                            uint64_t* v, uint64_t* sel, FinLds& L) {
-  const uint32_t nwin = c.nwin, ncand = c.ncand, cnt = nwin + ncand, krem = c.krem;
-  if (threadIdx.x == 0) {
-    L.ectr = 0;
-    L.vb[0] = ~0ull;
-    L.vb[1] = 0ull;
-  }
-  uint64_t a = ~0ull, o = 0ull;
-  for (uint32_t b = threadIdx.x; b < cnt; b += BLOCK * LOADS_IN_FLIGHT) {
-    uint64_t xs[LOADS_IN_FLIGHT];
-#pragma unroll
-    for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
-      const uint32_t i = b + (uint32_t)u * BLOCK;
-      xs[u] = (i < cnt) ? src[i] : 0ull;
-    }
-#pragma unroll
-    for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
-      const uint32_t i = b + (uint32_t)u * BLOCK;
-      if (i < cnt) {
-        v[i] = xs[u];
-        if (i >= nwin) {
-          a &= xs[u];
-          o |= xs[u];
-        }
-      }
-    }
-  }
-  __syncthreads();  // L initialised before the atomics below
-  block_and_or(a, o, L.vb);
-  uint64_t P = c.b_cand;
-  int s = 52;
-  if (krem > 0 && krem < ncand)
-    s = select_digits<true>(LdsSrc{v + nwin}, ncand, 52, P, krem, (uint64_t)(L.vb[0] ^ L.vb[1]),
-                            (uint64_t)L.vb[0], L.lh, L.wsum, L.res);
-  for (uint32_t i = threadIdx.x; i < nwin; i += BLOCK) sel[i] = v[i];
-  for (uint32_t i = threadIdx.x; i < ncand; i += BLOCK) {
-    const uint64_t x = v[nwin + i];
-    const bool take = (x >> s) <= P;
-    const uint64_t tm = ballot(take);
-    if (tm) {
-      const uint32_t slot = wave_append(take, tm, &L.ectr);
-      if (take && slot < krem) sel[nwin + slot] = x;
-    }
-  }
-  __syncthreads();  // sel complete; v is free for the sort's scratch
-  sort_selected<EMAX>(d, sel, c.m, v);
-}
-
 template <int EMAX>
 __device__ __forceinline__ void sort_any(const TaskDesc& d, uint32_t m, uint64_t* xs) {
   if constexpr (EMAX > 1) {
@@ -1161,16 +944,11 @@ __device__ __forceinline__ void sort_any(const TaskDesc& d, uint32_t m, uint64_t
   sort_task_regs<EMAX>(d, m, xs);
 }
 
-// Largest nwin + ncand finished by k_sort (a power of two >= BLOCK) and the
-// register blocking that sorts that many.
-constexpr uint32_t FIN_CAP = 2048;
-constexpr int FIN_E = (int)(FIN_CAP / BLOCK);
-
 // ---------------------------------------------------------------------------
-// Final sort, one block per task.  Tasks finished here (fin) sort their
-// compacted winners and candidates; the others sort the m results staged in
-// their output slots: registers + LDS up to BLOCK * E results, else a bitonic
-// in the task's (now free) spill region, which holds len >= m slots.
+// Final sort, one block per task, for launches whose results exceed
+// k_finish's LDS: sorts the m results staged in the output slots, in
+// registers + LDS up to BLOCK * E results, else by a bitonic in the task's
+// (now free) spill region, which holds len >= m slots.
 // ---------------------------------------------------------------------------
 struct GlobalPtr {
   uint64_t* __restrict__ p;
@@ -1184,19 +962,11 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
                                                 const TaskCtl* __restrict__ ctl,
                                                 uint64_t* __restrict__ cand) {
   extern __shared__ uint64_t sb[];
-  __shared__ FinLds fl;
   const int t = blockIdx.x;
   const TaskDesc d = tasks[t];
-  const TaskCtl c = ctl[t];
-  const uint32_t m = c.m;
+  const uint32_t m = ctl[t].m;
   const bool i64 = (d.flags & F_IDX64) != 0;
-  if (c.fin) {
-    // k_select admits only nwin + ncand <= fin_cap; the dynamic LDS holds
-    // fin_cap entries of v, then sel
-    uint32_t n = c.spill_all ? c.cand_ctr : 0u;
-    if (n > (uint32_t)d.len) n = (uint32_t)d.len;
-    finish_task<FIN_E>(d, c, cand + d.base + n, sb, sb + FIN_CAP, fl);
-  } else if (m <= (uint32_t)BLOCK * E) {
+  if (m <= (uint32_t)BLOCK * E) {
     sort_any<E>(d, m, sb);
   } else {
     uint64_t* __restrict__ g = cand + d.base;
@@ -1300,6 +1070,196 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Finish, one block per long task, after pass A and its selection (and pass B
+// over several blocks for tasks longer than big_len): gathers the task's
+// winners and candidates from its spill, or re-reads the task itself when it
+// has no usable spill (len <= big_len), resolves the krem best candidates
+// digit by digit (rank bits [19:0], then positions for exact ties; the
+// reference's passes 2/3 and filter, algorithm.jl:247-330) and sorts the m
+// results into the outputs.
+//   fin tasks (nwin + ncand <= cap): winners and candidates in LDS (v),
+//     narrow digits; winners and chosen candidates gathered in sel and sorted
+//     in registers.
+//   others: winners to the output slots, candidates behind the spill (or at
+//     the start of the spill region after a re-read, or left among the spill
+//     when there is no room); candidates resolved in LDS when they fit cap,
+//     else in global memory with wide digits (the 4096-bin histogram lives in
+//     v); chosen ones emitted behind the winners; with sort_here the m output
+//     slots are then sorted in registers (else k_sort follows).
+// Dynamic LDS: cap entries (v), then min(max m, cap) entries (sel) when fin.
+// ---------------------------------------------------------------------------
+constexpr int FIN_UNR = 8;  // float4 loads in flight per thread when re-reading a task
+
+template <int EMAX>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8 ? 8 : 1))) void k_finish(const TaskDesc* __restrict__ tasks,
+                                                  TaskCtl* __restrict__ ctl,
+                                                  uint32_t* __restrict__ hist,
+                                                  uint64_t* __restrict__ cand, uint32_t cap, uint32_t fin_cap,
+                                                  int64_t big_len, int sort_here) {
+  extern __shared__ uint64_t v[];
+  __shared__ uint32_t lh8[256];
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t res[3];
+  __shared__ uint32_t wctr, cctr, ectr;
+  __shared__ unsigned long long vb[2];
+  __shared__ TaskCtl sc_ctl;
+  const int t = blockIdx.x;
+  const TaskDesc d = tasks[t];
+  TaskCtl c = ctl[t];
+  if (!c.done) {  // spread over several pass-A blocks: select from the global histogram
+    uint32_t h[16];
+    take_hist(hist + (size_t)t * HIST_BINS, h);
+    select_first(d, h, c.b_pred, c.cand_ctr, fin_cap, &sc_ctl, wsum, res);
+    __syncthreads();
+    c = sc_ctl;
+    if (threadIdx.x == 0) ctl[t] = c;  // k_sort reads m
+  }
+  const uint32_t m = c.m;
+  if (m == 0) {
+    if (sort_here)
+      for (uint32_t i = threadIdx.x; i < (uint32_t)d.k; i += BLOCK) write_slot(d, i, 0ull, false);
+    return;
+  }
+  const uint32_t nwin = c.nwin, ncand = c.ncand, krem = c.krem, bw = c.b_win, bc = c.b_cand;
+  const bool fin = c.fin != 0;
+  const uint32_t len = (uint32_t)d.len;
+  uint64_t* __restrict__ region = cand + d.base;
+  const uint32_t n_sp = c.cand_ctr < len ? c.cand_ctr : len;  // spill, or pass B's candidates
+  const bool reread_done = !c.spill_all && d.len > big_len;    // k_compact re-read the task
+  const bool behind = c.spill_all && (uint64_t)n_sp + ncand <= (uint64_t)len;
+  uint64_t* __restrict__ cdst = c.spill_all ? region + n_sp : region;
+  if (threadIdx.x == 0) {
+    wctr = 0;
+    cctr = 0;
+    ectr = 0;
+    vb[0] = ~0ull;
+    vb[1] = 0ull;
+  }
+  __syncthreads();
+  uint64_t a = ~0ull, o = 0ull;
+  // One composite (rank << 32 | position), by all active lanes together.
+  auto take = [&](uint64_t x, bool w, bool cd) {
+    const uint64_t wm = ballot(w);
+    if (wm) {
+      const uint32_t slot = wave_append(w, wm, &wctr);
+      if (w && slot < nwin) {
+        if (fin) {
+          v[slot] = x;
+        } else {
+          uint32_t pos;
+          const float val = spilled_value(d, x, pos);
+          put_result(d, slot, val, pos);
+        }
+      }
+    }
+    const uint64_t cm = ballot(cd);
+    if (cm) {
+      if (cd) {
+        a &= x;
+        o |= x;
+      }
+      if (fin || behind || !c.spill_all) {
+        const uint32_t slot = wave_append(cd, cm, &cctr);
+        if (cd && slot < ncand) {
+          if (fin) v[nwin + slot] = x;
+          else cdst[slot] = x;
+        }
+      }
+    }
+  };
+  if (c.spill_all) {
+    for_each_entry(GlobalSrc{region}, n_sp, [&](uint64_t x) {
+      const uint32_t dg = (uint32_t)(x >> 52);
+      take(x, dg < bw, dg == bc);
+    });
+  } else if (!reread_done) {
+    const float sc = task_scaler(d);
+    const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
+    const uint32_t rmax = digit_bound(bc != NO_BIN ? bc : bw - 1);
+    stream_range<FIN_UNR>(d.in, 0, d.len, [&](float x, uint32_t pos) {
+      const uint32_t rk = rank_key(x, sc, xm);
+      const bool hit = rk <= rmax;
+      if (ballot(hit)) {
+        const uint32_t dg = rk >> 20;
+        take(((uint64_t)rk << 32) | pos, hit && dg < bw, hit && dg == bc);
+      }
+    });
+  } else if (fin) {  // k_compact left winners, then candidates, in the spill region
+    const uint32_t cnt = nwin + ncand;
+    for (uint32_t i = threadIdx.x; i < cnt; i += BLOCK) {
+      const uint64_t x = region[i];
+      v[i] = x;
+      if (i >= nwin) {
+        a &= x;
+        o |= x;
+      }
+    }
+  } else {  // k_compact emitted the winners and left the candidates in the spill region
+    for_each_entry(GlobalSrc{region}, n_sp, [&](uint64_t x) {
+      a &= x;
+      o |= x;
+    });
+  }
+  __syncthreads();
+  block_and_or(a, o, vb);
+  const uint64_t vary = (uint64_t)(vb[0] ^ vb[1]), same = (uint64_t)vb[0];
+  uint64_t P = bc;
+  int s = 52;
+  if (fin) {
+    if (krem > 0 && krem < ncand)
+      s = select_digits<true>(LdsSrc{v + nwin}, ncand, 52, P, krem, vary, same, lh8, wsum, res);
+    uint64_t* sel = v + cap;
+    for (uint32_t i = threadIdx.x; i < nwin; i += BLOCK) sel[i] = v[i];
+    for (uint32_t i = threadIdx.x; i < ncand; i += BLOCK) {
+      const uint64_t x = v[nwin + i];
+      const bool tk = (x >> s) <= P;
+      const uint64_t tm = ballot(tk);
+      if (tm) {
+        const uint32_t slot = wave_append(tk, tm, &ectr);
+        if (tk && slot < krem) sel[nwin + slot] = x;
+      }
+    }
+    __syncthreads();  // sel complete; v is free for the sort's scratch
+    sort_selected<EMAX>(d, sel, m, v);
+    return;
+  }
+  if (krem > 0) {
+    // the candidates: compacted (cdst / the re-read's list) or among the spill
+    const bool mixed = c.spill_all && !behind;
+    const uint64_t* __restrict__ src = mixed ? region : cdst;
+    const uint32_t nsrc = (mixed || reread_done) ? n_sp : ncand;
+    const bool in_lds = nsrc <= cap;
+    if (in_lds) {
+      for (uint32_t i = threadIdx.x; i < nsrc; i += BLOCK) v[i] = src[i];
+      __syncthreads();
+    }
+    if (krem < ncand) {
+      if (in_lds) s = select_digits<true>(LdsSrc{v}, nsrc, 52, P, krem, vary, same, lh8, wsum, res);
+      else s = select_digits<false>(GlobalSrc{src}, nsrc, 52, P, krem, vary, same, (uint32_t*)v, wsum, res);
+    }
+    auto emit = [&](uint64_t x) {
+      const bool tk = (x >> 52) == bc && (x >> s) <= P;
+      const uint64_t tm = ballot(tk);
+      if (tm) {
+        const uint32_t slot = wave_append(tk, tm, &ectr);
+        if (tk && slot < krem) {
+          uint32_t pos;
+          const float val = spilled_value(d, x, pos);
+          put_result(d, nwin + slot, val, pos);
+        }
+      }
+    };
+    if (in_lds) for_each_entry(LdsSrc{v}, nsrc, emit);
+    else for_each_entry(GlobalSrc{src}, nsrc, emit);
+  }
+  if (sort_here) {
+    __threadfence_block();
+    __syncthreads();  // the block's output-slot writes, and v free
+    sort_any<EMAX>(d, m, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Short tasks: one block does the whole top-k in LDS.  Dynamic LDS holds
 // 2 * cap composites (the task, then the chosen ones).
 // ---------------------------------------------------------------------------
@@ -1396,14 +1356,23 @@ __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ s
 
 // Explicit instantiations used by the host.
 #define RADIK_INST(U)                                                                                   \
-  template __global__ void k_hist<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, uint32_t*, \
-                                     TaskCtl*, uint64_t*, const uint32_t*);                             \
+  template __global__ void k_hist<U, true>(const TaskDesc*, const int64_t*, int, int64_t, int64_t,      \
+                                           uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t);  \
+  template __global__ void k_hist<U, false>(const TaskDesc*, const int64_t*, int, int64_t, int64_t,     \
+                                            uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t); \
   template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, int64_t,   \
-                                        TaskCtl*, uint64_t*, const uint32_t*);
+                                        TaskCtl*, uint64_t*, const uint32_t*, int64_t);
 RADIK_INST(1)
 RADIK_INST(2)
 RADIK_INST(4)
 #undef RADIK_INST
+template __global__ void k_finish<1>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
+template __global__ void k_finish<2>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
+template __global__ void k_finish<4>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
+template __global__ void k_finish<8>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
+template __global__ void k_finish<16>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
+template __global__ void k_finish<32>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
+template __global__ void k_finish<64>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
 template __global__ void k_sort<1>(const TaskDesc*, const TaskCtl*, uint64_t*);
 template __global__ void k_sort<2>(const TaskDesc*, const TaskCtl*, uint64_t*);
 template __global__ void k_sort<4>(const TaskDesc*, const TaskCtl*, uint64_t*);

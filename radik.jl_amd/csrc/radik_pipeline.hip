@@ -52,8 +52,14 @@ int dev_facts(DevFacts** out) {
     if (budget > 65536) {
       // Opt the per-task kernels into the larger allocation.
       const int want = (int)budget;
-      bool ok = hipFuncSetAttribute((const void*)k_refine, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    want - 20 * 1024) == hipSuccess &&
+      const int fin_want = want - (int)FINISH_STATIC_LDS;
+      bool ok = hipFuncSetAttribute((const void*)k_finish<1>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_finish<2>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_finish<4>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_finish<8>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_finish<16>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_finish<32>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
+                hipFuncSetAttribute((const void*)k_finish<64>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
                 hipFuncSetAttribute((const void*)k_sort<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     want) == hipSuccess &&
                 hipFuncSetAttribute((const void*)k_sort<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -66,10 +72,10 @@ int dev_facts(DevFacts** out) {
       }
     }
     f.lds_per_block = budget;
-    const size_t stat = 20 * 1024;  // static LDS of k_refine / k_small (histogram + scratch)
-    f.refine_cap_max = (uint32_t)((budget - stat) / 8);
+    const size_t stat = 20 * 1024;  // static LDS of k_small (histogram + scratch)
+    f.refine_cap_max = (uint32_t)((budget - FINISH_STATIC_LDS) / 8);
     f.small_cap_max = (uint32_t)((budget - stat) / 16);
-    // Defaults keep k_refine and k_small at two or more blocks per CU.
+    // Defaults keep k_finish (many tasks) and k_small at two or more blocks per CU.
     f.refine_cap = std::min<uint32_t>(4096, f.refine_cap_max);
     f.small_cap = std::min<uint32_t>(2048, f.small_cap_max);
     // register-blocked sort: up to 64 per thread, LDS scratch of a power of two
@@ -94,10 +100,6 @@ Pipeline::~Pipeline() {
   if (d_cbases) (void)hipFree(d_cbases);
   if (h_mbases) (void)hipHostFree(h_mbases);
   if (d_mbases) (void)hipFree(d_mbases);
-  if (h_xbases) (void)hipHostFree(h_xbases);
-  if (d_xbases) (void)hipFree(d_xbases);
-  if (h_xmap) (void)hipHostFree(h_xmap);
-  if (d_xmap) (void)hipFree(d_xmap);
   if (h_bmap) (void)hipHostFree(h_bmap);
   if (d_bmap) (void)hipFree(d_bmap);
   if (d_shist) (void)hipFree(d_shist);
@@ -120,7 +122,7 @@ static size_t grow(size_t have, size_t need) {
   return c;
 }
 
-int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, int64_t nx, hipStream_t s) {
+int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
   if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
   if (!side) {  // default priority, like every stream of this library
     RADIK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
@@ -138,26 +140,20 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, int64_t nx, hipStream
     if (h_sbases) RADIK_HIP(hipHostFree(h_sbases));
     if (h_cbases) RADIK_HIP(hipHostFree(h_cbases));
     if (h_mbases) RADIK_HIP(hipHostFree(h_mbases));
-    if (h_xbases) RADIK_HIP(hipHostFree(h_xbases));
     if (d_large) RADIK_HIP(hipFree(d_large));
     if (d_bases) RADIK_HIP(hipFree(d_bases));
     if (d_sbases) RADIK_HIP(hipFree(d_sbases));
     if (d_cbases) RADIK_HIP(hipFree(d_cbases));
     if (d_mbases) RADIK_HIP(hipFree(d_mbases));
-    if (d_xbases) RADIK_HIP(hipFree(d_xbases));
     if (d_ctl) RADIK_HIP(hipFree(d_ctl));
     h_large = nullptr; h_bases = nullptr; h_sbases = nullptr; h_cbases = nullptr; h_mbases = nullptr;
-    h_xbases = nullptr;
     d_large = nullptr; d_bases = nullptr; d_sbases = nullptr; d_cbases = nullptr; d_mbases = nullptr;
-    d_xbases = nullptr;
     d_ctl = nullptr;
     if (hipHostMalloc((void**)&h_large, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_bases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_sbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_cbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_mbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_xbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&d_xbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_cbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_mbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc((void**)&d_large, c * sizeof(TaskDesc)) != hipSuccess ||
@@ -198,19 +194,6 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, int64_t nx, hipStream
     }
     cap_small = c;
   }
-  if (!h_xmap || (size_t)nx > cap_xmap) {
-    RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = grow(cap_xmap, (size_t)nx);
-    if (h_xmap) RADIK_HIP(hipHostFree(h_xmap));
-    if (d_xmap) RADIK_HIP(hipFree(d_xmap));
-    h_xmap = nullptr; d_xmap = nullptr;
-    if (hipHostMalloc((void**)&h_xmap, c * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&d_xmap, c * sizeof(uint32_t)) != hipSuccess) {
-      cap_xmap = 0;
-      return set_error(6, "allocation of a %zu-entry split map failed", c);
-    }
-    cap_xmap = c;
-  }
   if ((size_t)tot > cap_cand) {
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = (size_t)tot;
@@ -227,8 +210,9 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, int64_t nx, hipStream
 
 // Sample groups (of SAMPLE_GROUP elements) read to predict a long task's
 // first-digit bound: about 1/64 of the task, 16..1024 groups.  Tasks shorter
-// than 2^16 elements, or whose k exceeds 1/64 of their length, are not
-// predicted and take the two-read schedule.
+// than 2^16 elements (re-reading them costs less than sampling), or whose k
+// exceeds 1/64 of their length, are not predicted: k_finish (or pass B)
+// re-reads them.
 static int64_t sample_groups(int64_t len, int64_t k) {
   const int64_t m = k < len ? k : len;
   if (len < (int64_t)1 << 16 || m <= 0 || m * 64 > len) return 0;
@@ -250,13 +234,6 @@ static void fill_block_map(uint32_t* out, int grid, int64_t gbeg, int64_t chunk,
   }
 }
 
-// k_split blocks of a predicted task: one per 2^17 elements, 1..128 (the
-// spill grows with the task and with ties at the threshold).
-static int64_t split_blocks(int64_t len) {
-  const int64_t b = (len + ((int64_t)1 << 17) - 1) >> 17;
-  return b < 1 ? 1 : (b > 128 ? 128 : b);
-}
-
 // Short tasks run whole in one block (k_small); small_cap 0 routes every
 // task through the streaming passes.
 static inline bool is_short(int64_t len, uint32_t small_cap) {
@@ -268,22 +245,20 @@ static inline bool is_short(int64_t len, uint32_t small_cap) {
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
                 hipStream_t s) {
   const uint32_t scap = eff_small_cap(f, o);
-  int64_t nl = 0, ns = 0, tot = 0, tot_pred = 0, nx = 0;
+  int64_t nl = 0, ns = 0, tot = 0, tot_pred = 0;
   for (int64_t t = 0; t < n; ++t) {
     if (is_short(tasks[t].len, scap)) {
       ++ns;
     } else {
       ++nl;
       tot += tasks[t].len;
-      if (o.predict && sample_groups(tasks[t].len, tasks[t].k)) {
-        tot_pred += tasks[t].len;
-        nx += split_blocks(tasks[t].len);
-      }
+      if (o.predict && sample_groups(tasks[t].len, tasks[t].k)) tot_pred += tasks[t].len;
     }
   }
-  int rc = p.reserve(nl, ns, tot, nx, s);
+  int rc = p.reserve(nl, ns, tot, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0, runs = 0, msum = 0, npred = 0, xsum = 0;
+  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0, runs = 0, msum = 0;
+  bool big = false;
   // pass 0: predicted long tasks; pass 1: short tasks and the other long tasks
   for (int pass = 0; pass < 2; ++pass)
   for (int64_t t = 0; t < n; ++t) {
@@ -299,6 +274,8 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     d.len = ts.len;
     d.k = (int32_t)ts.k;
     d.flags = ts.flags;
+    d.nblk = 0;
+    d.spare = 0;
     if (shrt) {
       d.base = 0;
       p.h_small[is++] = d;
@@ -312,13 +289,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
       p.h_mbases[il] = msum;
       runs += std::max<int64_t>(1, (m + SORT_CHUNK - 1) / SORT_CHUNK);  // run 0 also pads
       msum += m;
-      if (pred) {
-        p.h_xbases[npred] = xsum;
-        const int64_t nb = split_blocks(ts.len);
-        for (int64_t j = 0; j < nb; ++j) p.h_xmap[xsum + j] = (uint32_t)npred;
-        xsum += nb;
-        ++npred;
-      }
+      if (ts.len > BIG_LEN) big = true;
       p.h_large[il++] = d;
       base += ts.len;
       groups += pred ? sample_groups(ts.len, ts.k) : 0;
@@ -328,19 +299,19 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.h_sbases[il] = groups;
   p.h_cbases[il] = runs;
   p.h_mbases[il] = msum;
-  p.h_xbases[npred] = xsum;
-  p.n_pred = npred;
-  p.total_split = xsum;
-  // Launch shapes of the streaming kernels and their block -> task maps.
-  p.grid_h = p.grid_d = p.grid_p = p.grid_s = 0;
+  p.has_big = big;
+  // Launch shapes of the streaming kernels and their block -> task maps;
+  // each long task learns how many pass-A blocks share it.
+  p.grid_h = p.grid_s = 0;
   p.chunk_h = nl ? stream_chunk(tot, f, o, &p.grid_h) : 0;
-  p.chunk_d = (tot > tot_pred) ? stream_chunk(tot - tot_pred, f, o, &p.grid_d) : 0;
-  p.chunk_p = (tot_pred > 0) ? stream_chunk(tot_pred, f, o, &p.grid_p) : 0;
   p.grid_s = groups > 0 ? (int)((groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK) : 0;
+  for (int64_t t = 0; t < nl; ++t) {
+    const int64_t b0 = p.h_bases[t], len = p.h_large[t].len;
+    p.h_large[t].nblk = len > 0 ? (int32_t)((b0 + len - 1) / p.chunk_h - b0 / p.chunk_h + 1) : 0;
+    p.h_large[t].spare = 0;
+  }
   p.off_h = 0;
-  p.off_d = p.off_h + p.grid_h;
-  p.off_p = p.off_d + p.grid_d;
-  p.off_s = p.off_p + p.grid_p;
+  p.off_s = p.off_h + p.grid_h;
   p.n_bmap = p.off_s + p.grid_s;
   if (!p.h_bmap || (size_t)p.n_bmap > p.cap_bmap) {
     RADIK_HIP(hipStreamSynchronize(s));
@@ -358,8 +329,6 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     p.cap_bmap = c;
   }
   fill_block_map(p.h_bmap + p.off_h, p.grid_h, 0, p.chunk_h, p.h_bases, nl);
-  fill_block_map(p.h_bmap + p.off_d, p.grid_d, tot_pred, p.chunk_d, p.h_bases, nl);
-  fill_block_map(p.h_bmap + p.off_p, p.grid_p, 0, p.chunk_p, p.h_bases, nl);
   fill_block_map(p.h_bmap + p.off_s, p.grid_s, 0, SAMPLE_GROUPS_PER_BLOCK, p.h_sbases, nl);
   p.total_runs = runs;
   p.total_m = msum;
@@ -379,9 +348,6 @@ int upload(Pipeline& p, hipStream_t s) {
     RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_cbases, p.h_cbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     RADIK_HIP(hipMemcpyAsync(p.d_mbases, p.h_mbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    RADIK_HIP(hipMemcpyAsync(p.d_xbases, p.h_xbases, (p.n_pred + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    if (p.total_split)
-      RADIK_HIP(hipMemcpyAsync(p.d_xmap, p.h_xmap, p.total_split * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     if (p.n_bmap)
       RADIK_HIP(hipMemcpyAsync(p.d_bmap, p.h_bmap, p.n_bmap * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   }
@@ -394,17 +360,19 @@ int upload(Pipeline& p, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Kernel sequence
 // ---------------------------------------------------------------------------
+// Pass A with the fused selection (fin_cap: see k_finish).
 template <int UNR>
-static void launch_stream_kernels_hist(const Pipeline& p, hipStream_t s) {
-  hipLaunchKernelGGL(k_hist<UNR>, dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     p.total, p.chunk_h, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h);
+static void launch_stream_kernels_hist(const Pipeline& p, uint32_t fin_cap, hipStream_t s) {
+  hipLaunchKernelGGL((k_hist<UNR, true>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
+                     (int)p.n_large, p.total, p.chunk_h, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
 }
 
+// Pass B over the whole element space (pass A's chunks); blocks skip tasks of
+// at most BIG_LEN elements and those with a usable spill.
 template <int UNR>
-static void launch_stream_kernels_compact(const Pipeline& p, int64_t g_begin, int64_t g_end, int64_t chunk,
-                                          int grid, const uint32_t* bmap, hipStream_t s) {
-  hipLaunchKernelGGL(k_compact<UNR>, dim3(grid), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     g_begin, g_end, chunk, p.d_ctl, p.d_cand, bmap);
+static void launch_stream_kernels_compact(const Pipeline& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_compact<UNR>, dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
+                     (int64_t)0, p.total, p.chunk_h, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN);
 }
 
 // Equal chunks of `total` elements, about blocks_per_cu blocks per CU.
@@ -450,80 +418,98 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
                        p.d_ctl);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
+    // k_finish's LDS: cap entries for a task's winners + candidates (a power
+    // of two >= 2048, which also holds the 4096-bin histogram of the wide
+    // digits), plus the m results of fin tasks; results sorted in k_finish
+    // when every task's m fits cap (else k_sort follows, without fin tasks).
+    const uint32_t budget = (uint32_t)((f.lds_per_block - FINISH_STATIC_LDS) / 8);
+    uint32_t cap = 2048;
+    {
+      const uint32_t want = (p.n_large <= (int64_t)f.cus) ? 16384u : std::max<uint32_t>(2048u, eff_refine_cap(f, o));
+      while (cap * 2 <= want && cap * 2 <= budget) cap *= 2;
+    }
+    bool sort_here = false;
+    uint32_t sel = 0;
+    for (;;) {
+      sort_here = p.max_m <= cap && p.max_m <= (uint32_t)BLOCK * 64;
+      sel = sort_here ? std::min<uint32_t>(p.max_m, cap) : 0u;
+      if (cap + sel <= budget || cap <= 2048) break;
+      cap /= 2;
+    }
+    if (cap + sel > budget) {
+      sort_here = false;
+      sel = 0;
+    }
+    const uint32_t fin_cap = sort_here ? cap : 0u;
+    uint32_t emax = 1;
+    while (sort_here && (uint32_t)BLOCK * emax < p.max_m) emax *= 2;
     switch (o.unr) {
-      case 1: launch_stream_kernels_hist<1>(p, s); break;
-      case 4: launch_stream_kernels_hist<4>(p, s); break;
-      default: launch_stream_kernels_hist<2>(p, s); break;
+      case 1: launch_stream_kernels_hist<1>(p, fin_cap, s); break;
+      case 4: launch_stream_kernels_hist<4>(p, fin_cap, s); break;
+      default: launch_stream_kernels_hist<2>(p, fin_cap, s); break;
     }
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    // Register sort blocking of k_sort: the launch's largest m, and room for
-    // the tasks it finishes from their compacted winners and candidates
-    // (not with the multi-block sort, which reads staged output slots).
-    const bool multi_sort = p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS;
-    uint32_t e = 1;
-    while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;
-    // Finished tasks keep nwin + ncand <= fin_cap <= FIN_CAP entries and
-    // their m results in LDS: FIN_CAP + min(max_m, FIN_CAP) entries.
-    uint32_t fin_cap = 0;
-    size_t sort_lds = 0;
-    if (!multi_sort && o.fin_cap > 0) {
-      fin_cap = std::min<uint32_t>(o.fin_cap, FIN_CAP);
-      sort_lds = ((size_t)FIN_CAP + std::min<uint32_t>(p.max_m, FIN_CAP)) * 8;
+    // Tasks spread over several pass-A blocks are selected by k_finish, or
+    // before pass B over several blocks (needed only for long tasks without a
+    // usable spill; k_finish re-reads the shorter ones).
+    if (p.has_big) {
+      hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl,
+                         fin_cap);
+      RADIK_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl,
-                       fin_cap);
-    RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    // Pass B: the tasks without a prediction (all of them re-read), then the
-    // predicted ones (blocks skip tasks whose spill covered the bin).
-    for (int part = 0; part < 2; ++part) {
-      const int64_t gb = part == 0 ? p.total_pred : 0;
-      const int64_t ge = part == 0 ? p.total : p.total_pred;
-      if (ge <= gb) continue;
-      const int64_t c2 = part == 0 ? p.chunk_d : p.chunk_p;
-      const int g2 = part == 0 ? p.grid_d : p.grid_p;
-      const uint32_t* bm = p.d_bmap + (part == 0 ? p.off_d : p.off_p);
+    if (p.has_big) {
       switch (o.unr) {
-        case 1: launch_stream_kernels_compact<1>(p, gb, ge, c2, g2, bm, s); break;
-        case 4: launch_stream_kernels_compact<4>(p, gb, ge, c2, g2, bm, s); break;
-        default: launch_stream_kernels_compact<2>(p, gb, ge, c2, g2, bm, s); break;
+        case 1: launch_stream_kernels_compact<1>(p, s); break;
+        case 4: launch_stream_kernels_compact<4>(p, s); break;
+        default: launch_stream_kernels_compact<2>(p, s); break;
       }
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
-    if (p.n_pred > 0) {
-      hipLaunchKernelGGL(k_split, dim3((unsigned)p.total_split), dim3(BLOCK), 0, s, p.d_large, p.d_xbases,
-                         p.d_xmap, p.d_ctl, p.d_cand);
-      RADIK_HIP(hipGetLastError());
-    }
-    const uint32_t rcap = eff_refine_cap(f, o);
-    hipLaunchKernelGGL(k_refine, dim3((unsigned)p.n_large), dim3(BLOCK), (size_t)rcap * 8, s, p.d_large,
-                       p.d_ctl, p.d_cand, rcap);
-    RADIK_HIP(hipGetLastError());
-    if ((rc = mark())) return rc;
-    if (multi_sort) {
-      // several blocks per task: sorted runs, then ranks across runs
-      hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)p.total_runs), dim3(BLOCK), 0, s, p.d_large, p.d_ctl,
-                         p.d_cbases, (int)p.n_large, p.d_cand);
-      RADIK_HIP(hipGetLastError());
-      const unsigned rg = (unsigned)((p.total_m + BLOCK - 1) / BLOCK);
-      hipLaunchKernelGGL(k_sort_rank, dim3(rg), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_mbases,
-                         (int)p.n_large, p.total_m, (const uint64_t*)p.d_cand);
-    } else {
-      const size_t lds = std::max<size_t>((size_t)BLOCK * e * 8, sort_lds);
+    {
+      const size_t lds = ((size_t)cap + sel) * 8;
       const dim3 g((unsigned)p.n_large), b(BLOCK);
-      switch (e) {
-        case 1: hipLaunchKernelGGL(k_sort<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-        case 2: hipLaunchKernelGGL(k_sort<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-        case 4: hipLaunchKernelGGL(k_sort<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-        case 8: hipLaunchKernelGGL(k_sort<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-        case 16: hipLaunchKernelGGL(k_sort<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-        case 32: hipLaunchKernelGGL(k_sort<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-        default: hipLaunchKernelGGL(k_sort<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+      const int sh = sort_here ? 1 : 0;
+      switch (emax) {
+        case 1: hipLaunchKernelGGL(k_finish<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
+        case 2: hipLaunchKernelGGL(k_finish<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
+        case 4: hipLaunchKernelGGL(k_finish<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
+        case 8: hipLaunchKernelGGL(k_finish<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
+        case 16: hipLaunchKernelGGL(k_finish<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
+        case 32: hipLaunchKernelGGL(k_finish<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
+        default: hipLaunchKernelGGL(k_finish<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
       }
+      RADIK_HIP(hipGetLastError());
     }
-    RADIK_HIP(hipGetLastError());
+    if ((rc = mark())) return rc;
+    if (!sort_here) {
+      if (p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS) {
+        // several blocks per task: sorted runs, then ranks across runs
+        hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)p.total_runs), dim3(BLOCK), 0, s, p.d_large, p.d_ctl,
+                           p.d_cbases, (int)p.n_large, p.d_cand);
+        RADIK_HIP(hipGetLastError());
+        const unsigned rg = (unsigned)((p.total_m + BLOCK - 1) / BLOCK);
+        hipLaunchKernelGGL(k_sort_rank, dim3(rg), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_mbases,
+                           (int)p.n_large, p.total_m, (const uint64_t*)p.d_cand);
+      } else {
+        uint32_t e = 1;
+        while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;
+        const size_t lds = (size_t)BLOCK * e * 8;
+        const dim3 g((unsigned)p.n_large), b(BLOCK);
+        switch (e) {
+          case 1: hipLaunchKernelGGL(k_sort<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+          case 2: hipLaunchKernelGGL(k_sort<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+          case 4: hipLaunchKernelGGL(k_sort<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+          case 8: hipLaunchKernelGGL(k_sort<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+          case 16: hipLaunchKernelGGL(k_sort<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+          case 32: hipLaunchKernelGGL(k_sort<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+          default: hipLaunchKernelGGL(k_sort<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
+        }
+      }
+      RADIK_HIP(hipGetLastError());
+    }
   } else {
     for (int i = 0; i < 6; ++i)
       if ((rc = mark())) return rc;
@@ -569,7 +555,8 @@ int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel
   hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
                      p.d_ctl);
   RADIK_HIP(hipGetLastError());
-  launch_stream_kernels_hist<2>(p, s);
+  hipLaunchKernelGGL((k_hist<2, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
+                     (int)p.n_large, p.total, p.chunk_h, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, 0u);
   RADIK_HIP(hipGetLastError());
   RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),
                            hipMemcpyDeviceToDevice, s));
