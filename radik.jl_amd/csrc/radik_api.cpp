@@ -120,7 +120,7 @@ int radik_ws_create(radik_ws** ws, int64_t n, int64_t num_tasks, int counter_byt
   // is sized on first use from the call's lengths.
   const int64_t nl = (n > (int64_t)f->small_cap) ? num_tasks : 0;
   const int64_t ns = (n <= (int64_t)f->small_cap) ? num_tasks : 0;
-  rc = w->pipe.reserve(nl, ns, 0, (hipStream_t)stream);
+  rc = w->pipe.reserve(nl, ns, 0, 0, (hipStream_t)stream);
   if (rc) return rc;
   *ws = w.release();
   return 0;
@@ -443,6 +443,7 @@ int radik_diag_pass1(const float* val_in, const int64_t* task_lens, int64_t num_
   hipStream_t s = (hipStream_t)stream;
   LaunchOpts o;
   o.predict = false;
+  o.medium = false;
   rc = stage_tasks(pipe, specs.data(), num_tasks, all, o, s);
   if (rc) return rc;
   rc = upload(pipe, s);

@@ -40,6 +40,10 @@ int dev_facts(DevFacts** out);
 constexpr int64_t BIG_LEN = (int64_t)1 << 20;
 // Static LDS of k_finish beside its dynamic arrays (upper bound, bytes).
 constexpr size_t FINISH_STATIC_LDS = 2048;
+// Medium tasks (longer than the short cap, at most MED_LEN elements, not
+// sampled, m <= MED_CAP) run whole in one k_medium block beside the passes.
+constexpr int64_t MED_LEN = (int64_t)1 << 16;
+constexpr uint32_t MED_CAP = 2048;
 
 // Launch-level options.
 struct LaunchOpts {
@@ -48,6 +52,7 @@ struct LaunchOpts {
   uint32_t small_cap = 0;   // longest task for k_small (0: device default)
   uint32_t refine_cap = 0;  // candidates held in LDS by k_refine (0: device default)
   bool predict = true;      // sample long tasks and spill in pass A
+  bool medium = true;       // medium tasks to k_medium (else the streaming passes)
 };
 inline uint32_t eff_small_cap(const DevFacts& f, const LaunchOpts& o) {
   return o.small_cap ? (o.small_cap < f.small_cap_max ? o.small_cap : f.small_cap_max) : f.small_cap;
@@ -67,6 +72,7 @@ struct Pipeline {
   int64_t* h_mbases = nullptr;  // result-count prefix sums (multi-block sort)
   uint32_t* h_bmap = nullptr;   // first task of each block of the streaming launches
   TaskDesc* h_small = nullptr;
+  TaskDesc* h_med = nullptr;
   // device
   TaskDesc* d_large = nullptr;
   int64_t* d_bases = nullptr;
@@ -75,11 +81,12 @@ struct Pipeline {
   int64_t* d_mbases = nullptr;
   uint32_t* d_bmap = nullptr;
   TaskDesc* d_small = nullptr;
+  TaskDesc* d_med = nullptr;
   TaskCtl* d_ctl = nullptr;
   uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
   uint32_t* d_shist = nullptr;  // sample histograms, zero between launches
   uint64_t* d_cand = nullptr;   // spill regions
-  size_t cap_large = 0, cap_small = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0;
+  size_t cap_large = 0, cap_small = 0, cap_med = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0;
   // Streaming launch shapes fixed at staging (passes A and B share one
   // chunking; sampling has its own): chunk, grid, offset of the block map.
   int64_t chunk_h = 0;
@@ -87,7 +94,8 @@ struct Pipeline {
   int64_t off_h = 0, off_s = 0, n_bmap = 0;
   // current content
   int64_t n_large = 0, n_small = 0, total = 0, total_groups = 0;
-  int64_t total_pred = 0;  // elements of the predicted tasks, which come first
+  int64_t n_med = 0;
+  uint32_t max_m_med = 0;
   bool has_big = false;    // some long task exceeds BIG_LEN (pass B over several blocks)
   uint32_t max_m = 0;  // largest min(k, len) over the long tasks
   int64_t total_runs = 0, total_m = 0;  // multi-block sort sizes
@@ -97,7 +105,8 @@ struct Pipeline {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   ~Pipeline();
-  int reserve(int64_t n_large, int64_t n_small, int64_t total_large, hipStream_t s);
+  // total_cand: spill-region entries (long and medium tasks)
+  int reserve(int64_t n_large, int64_t n_small, int64_t n_med, int64_t total_cand, hipStream_t s);
 };
 
 // Splits tasks into short ones (k_small) and long ones (streaming passes),

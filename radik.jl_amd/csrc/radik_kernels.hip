@@ -1090,30 +1090,47 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict_
 // ---------------------------------------------------------------------------
 constexpr int FIN_UNR = 8;  // float4 loads in flight per thread when re-reading a task
 
+// The q-th smallest (1 <= q <= n) of n distinct composites v[0, n) in LDS
+// that share the first digit `top`: the largest of the q smallest, which
+// the narrow digits isolate exactly.
+__device__ __forceinline__ uint64_t kth_smallest(const uint64_t* v, uint32_t n, uint32_t q, uint32_t top,
+                                                 uint64_t vary, uint64_t same, uint32_t* lh8, uint32_t* wsum,
+                                                 uint32_t* res, unsigned long long* mx) {
+  uint64_t P = top;
+  const int s = select_digits<true>(LdsSrc{v}, n, 52, P, q, vary, same, lh8, wsum, res);
+  if (threadIdx.x == 0) *mx = 0ull;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
+    if ((v[i] >> s) <= P) atomicMax(mx, (unsigned long long)v[i]);
+  __syncthreads();
+  const uint64_t r = *mx;
+  __syncthreads();
+  return r;
+}
+
+// LDS of a finishing block beside its dynamic entry array.
+struct FinShared {
+  uint32_t lh8[256];
+  uint32_t wsum[BLOCK / 64];
+  uint32_t res[3];
+  uint32_t wctr, cctr, ectr;
+  unsigned long long vb[2], mx;
+  TaskCtl ctl;
+};
+
+// Everything after the first-digit selection c of task d; see k_finish.
 template <int EMAX>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8 ? 8 : 1))) void k_finish(const TaskDesc* __restrict__ tasks,
-                                                  TaskCtl* __restrict__ ctl,
-                                                  uint32_t* __restrict__ hist,
-                                                  uint64_t* __restrict__ cand, uint32_t cap, uint32_t fin_cap,
-                                                  int64_t big_len, int sort_here) {
-  extern __shared__ uint64_t v[];
-  __shared__ uint32_t lh8[256];
-  __shared__ uint32_t wsum[BLOCK / 64];
-  __shared__ uint32_t res[3];
-  __shared__ uint32_t wctr, cctr, ectr;
-  __shared__ unsigned long long vb[2];
-  __shared__ TaskCtl sc_ctl;
-  const int t = blockIdx.x;
-  const TaskDesc d = tasks[t];
-  TaskCtl c = ctl[t];
-  if (!c.done) {  // spread over several pass-A blocks: select from the global histogram
-    uint32_t h[16];
-    take_hist(hist + (size_t)t * HIST_BINS, h);
-    select_first(d, h, c.b_pred, c.cand_ctr, fin_cap, &sc_ctl, wsum, res);
-    __syncthreads();
-    c = sc_ctl;
-    if (threadIdx.x == 0) ctl[t] = c;  // k_sort reads m
-  }
+__device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c, uint64_t* __restrict__ cand,
+                                            uint64_t* v, uint32_t cap, int64_t big_len, int sort_here,
+                                            FinShared& S) {
+  uint32_t* lh8 = S.lh8;
+  uint32_t* wsum = S.wsum;
+  uint32_t* res = S.res;
+  uint32_t& wctr = S.wctr;
+  uint32_t& cctr = S.cctr;
+  uint32_t& ectr = S.ectr;
+  unsigned long long* vb = S.vb;
+  unsigned long long& mx = S.mx;
   const uint32_t m = c.m;
   if (m == 0) {
     if (sort_here)
@@ -1167,7 +1184,122 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
       }
     }
   };
-  if (c.spill_all) {
+  // Many candidates of a spilled task: bound the krem-th smallest between two
+  // composites L <= U read off a sample of the spill (as pass A bounds the
+  // first digit), then one pass emits the winners and the candidates below
+  // L and keeps those in [L, U] in LDS; checked, else the exact digits below.
+  const bool sampled = !fin && c.spill_all && krem > 0 && ncand > cap;
+  bool exact = true;
+  if (sampled) {
+    const uint32_t want = cap < 4096u ? cap : 4096u;
+    const uint32_t stride = n_sp / want > 1u ? n_sp / want : 1u;
+    const uint32_t ns = (n_sp + stride - 1) / stride;  // sampled spill entries j * stride
+    for (uint32_t j0 = threadIdx.x; j0 < ns; j0 += BLOCK * LOADS_IN_FLIGHT) {
+      uint64_t xs[LOADS_IN_FLIGHT];
+#pragma unroll
+      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+        const uint32_t j = j0 + (uint32_t)u * BLOCK;
+        xs[u] = (j < ns) ? region[(size_t)j * stride] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+        const uint64_t x = xs[u];
+        const bool cd = (j0 + (uint32_t)u * BLOCK < ns) && (x >> 52) == bc;
+        const uint64_t cm = ballot(cd);
+        if (cm) {
+          const uint32_t slot = wave_append(cd, cm, &cctr);
+          if (cd && slot < cap) {
+            v[slot] = x;
+            a &= x;
+            o |= x;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    block_and_or(a, o, vb);
+    const uint32_t cs = cctr < cap ? cctr : cap;
+    const float mu = (float)krem * (float)cs / (float)ncand;
+    const float dev = 4.0f * sqrtf(mu) + 4.0f;
+    const int64_t qlo = (int64_t)floorf(mu - dev), qhi = (int64_t)ceilf(mu + dev);
+    const uint64_t sv = (uint64_t)(vb[0] ^ vb[1]), ss = (uint64_t)vb[0];
+    const uint64_t L = (cs > 0 && qlo >= 1) ? kth_smallest(v, cs, (uint32_t)qlo, bc, sv, ss, lh8, wsum, res, &mx) : 0ull;
+    const uint64_t U = (cs > 0 && qhi <= (int64_t)cs) ? kth_smallest(v, cs, (uint32_t)qhi, bc, sv, ss, lh8, wsum, res, &mx)
+                                                      : ~0ull;
+    if (threadIdx.x == 0) {
+      cctr = 0;
+      vb[0] = ~0ull;
+      vb[1] = 0ull;
+    }
+    __syncthreads();
+    a = ~0ull;
+    o = 0ull;
+    for_each_entry(GlobalSrc{region}, n_sp, [&](uint64_t x) {
+      const uint32_t dg = (uint32_t)(x >> 52);
+      const bool w = dg < bw;
+      const bool lo = dg == bc && x < L;
+      const bool mid = dg == bc && x >= L && x <= U;
+      const uint64_t wm = ballot(w);
+      if (wm) {
+        const uint32_t slot = wave_append(w, wm, &wctr);
+        if (w && slot < nwin) {
+          uint32_t pos;
+          const float val = spilled_value(d, x, pos);
+          put_result(d, slot, val, pos);
+        }
+      }
+      const uint64_t lm = ballot(lo);
+      if (lm) {
+        const uint32_t slot = wave_append(lo, lm, &ectr);
+        if (lo && slot < krem) {
+          uint32_t pos;
+          const float val = spilled_value(d, x, pos);
+          put_result(d, nwin + slot, val, pos);
+        }
+      }
+      const uint64_t mm = ballot(mid);
+      if (mm) {
+        const uint32_t slot = wave_append(mid, mm, &cctr);
+        if (mid && slot < cap) {
+          v[slot] = x;
+          a &= x;
+          o |= x;
+        }
+      }
+    });
+    __syncthreads();
+    const uint32_t below = ectr, nmid = cctr;
+    if (below <= krem && below + nmid >= krem && nmid <= cap) {
+      exact = false;
+      block_and_or(a, o, vb);
+      const uint32_t need = krem - below;
+      uint64_t Pm = bc;
+      int sm = 52;
+      if (need > 0 && need < nmid)
+        sm = select_digits<true>(LdsSrc{v}, nmid, 52, Pm, need, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh8,
+                                 wsum, res);
+      if (need > 0) {
+        for (uint32_t i = threadIdx.x; i < nmid; i += BLOCK) {
+          const uint64_t x = v[i];
+          const bool tk = (x >> sm) <= Pm;
+          const uint64_t tm = ballot(tk);
+          if (tm) {
+            const uint32_t slot = wave_append(tk, tm, &ectr);  // continues after the `below` ones
+            if (tk && slot < krem) {
+              uint32_t pos;
+              const float val = spilled_value(d, x, pos);
+              put_result(d, nwin + slot, val, pos);
+            }
+          }
+        }
+      }
+    } else {
+      // the sample misled: exact digits over the spill, every bit varying
+      a = 0ull;
+      o = ~0ull;
+      if (threadIdx.x == 0) ectr = 0;
+    }
+  } else if (c.spill_all) {
     for_each_entry(GlobalSrc{region}, n_sp, [&](uint64_t x) {
       const uint32_t dg = (uint32_t)(x >> 52);
       take(x, dg < bw, dg == bc);
@@ -1201,7 +1333,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
     });
   }
   __syncthreads();
-  block_and_or(a, o, vb);
+  if (exact) block_and_or(a, o, vb);
   const uint64_t vary = (uint64_t)(vb[0] ^ vb[1]), same = (uint64_t)vb[0];
   uint64_t P = bc;
   int s = 52;
@@ -1223,9 +1355,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
     sort_selected<EMAX>(d, sel, m, v);
     return;
   }
-  if (krem > 0) {
+  if (krem > 0 && exact) {
     // the candidates: compacted (cdst / the re-read's list) or among the spill
-    const bool mixed = c.spill_all && !behind;
+    const bool mixed = c.spill_all && (!behind || sampled);
     const uint64_t* __restrict__ src = mixed ? region : cdst;
     const uint32_t nsrc = (mixed || reread_done) ? n_sp : ncand;
     const bool in_lds = nsrc <= cap;
@@ -1257,6 +1389,57 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
     __syncthreads();  // the block's output-slot writes, and v free
     sort_any<EMAX>(d, m, v);
   }
+}
+
+template <int EMAX>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8 ? 8 : 1))) void k_finish(const TaskDesc* __restrict__ tasks,
+                                                  TaskCtl* __restrict__ ctl,
+                                                  uint32_t* __restrict__ hist,
+                                                  uint64_t* __restrict__ cand, uint32_t cap, uint32_t fin_cap,
+                                                  int64_t big_len, int sort_here) {
+  extern __shared__ uint64_t v[];
+  __shared__ FinShared S;
+  const int t = blockIdx.x;
+  const TaskDesc d = tasks[t];
+  TaskCtl c = ctl[t];
+  if (!c.done) {  // spread over several pass-A blocks: select from the global histogram
+    uint32_t h[16];
+    take_hist(hist + (size_t)t * HIST_BINS, h);
+    select_first(d, h, c.b_pred, c.cand_ctr, fin_cap, &S.ctl, S.wsum, S.res);
+    __syncthreads();
+    c = S.ctl;
+    if (threadIdx.x == 0) ctl[t] = c;  // k_sort reads m
+  }
+  finish_task<EMAX>(d, c, cand, v, cap, big_len, sort_here, S);
+}
+
+// ---------------------------------------------------------------------------
+// Medium tasks (short of sampling, too long for k_small), one block each,
+// beside the long-task passes: the first-digit histogram of a read of the
+// task in LDS (in the space of v, cap >= 2048 entries), the selection, then
+// finish_task re-reads the task while it is still cached.  Results sorted
+// here (m <= cap).
+// ---------------------------------------------------------------------------
+template <int EMAX>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_medium(const TaskDesc* __restrict__ tasks,
+                                                  uint64_t* __restrict__ cand, uint32_t cap) {
+  extern __shared__ uint64_t v[];
+  __shared__ FinShared S;
+  const TaskDesc d = tasks[blockIdx.x];
+  uint32_t* lh = reinterpret_cast<uint32_t*>(v);
+  for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
+  __syncthreads();
+  const float sc = task_scaler(d);
+  const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
+  stream_range<FIN_UNR>(d.in, 0, d.len, [&](float x, uint32_t) { atomicAdd(&lh[rank_key(x, sc, xm) >> 20], 1u); });
+  __syncthreads();
+  uint32_t h[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) h[j] = lh[threadIdx.x * 16 + j];
+  select_first(d, h, NO_BIN, 0u, cap, &S.ctl, S.wsum, S.res);
+  __syncthreads();  // the selection is published and the histogram read: v is free
+  const TaskCtl c = S.ctl;
+  finish_task<EMAX>(d, c, cand, v, cap, INT64_MAX, 1, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -1366,6 +1549,10 @@ RADIK_INST(1)
 RADIK_INST(2)
 RADIK_INST(4)
 #undef RADIK_INST
+template __global__ void k_medium<1>(const TaskDesc*, uint64_t*, uint32_t);
+template __global__ void k_medium<2>(const TaskDesc*, uint64_t*, uint32_t);
+template __global__ void k_medium<4>(const TaskDesc*, uint64_t*, uint32_t);
+template __global__ void k_medium<8>(const TaskDesc*, uint64_t*, uint32_t);
 template __global__ void k_finish<1>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
 template __global__ void k_finish<2>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
 template __global__ void k_finish<4>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);

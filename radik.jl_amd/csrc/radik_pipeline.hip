@@ -104,6 +104,8 @@ Pipeline::~Pipeline() {
   if (d_bmap) (void)hipFree(d_bmap);
   if (d_shist) (void)hipFree(d_shist);
   if (h_small) (void)hipHostFree(h_small);
+  if (h_med) (void)hipHostFree(h_med);
+  if (d_med) (void)hipFree(d_med);
   if (d_large) (void)hipFree(d_large);
   if (d_bases) (void)hipFree(d_bases);
   if (d_small) (void)hipFree(d_small);
@@ -122,7 +124,7 @@ static size_t grow(size_t have, size_t need) {
   return c;
 }
 
-int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
+int Pipeline::reserve(int64_t nl, int64_t ns, int64_t nm, int64_t tot, hipStream_t s) {
   if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
   if (!side) {  // default priority, like every stream of this library
     RADIK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
@@ -194,6 +196,19 @@ int Pipeline::reserve(int64_t nl, int64_t ns, int64_t tot, hipStream_t s) {
     }
     cap_small = c;
   }
+  if ((size_t)nm > cap_med) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = grow(cap_med, (size_t)nm);
+    if (h_med) RADIK_HIP(hipHostFree(h_med));
+    if (d_med) RADIK_HIP(hipFree(d_med));
+    h_med = nullptr; d_med = nullptr;
+    if (hipHostMalloc((void**)&h_med, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_med, c * sizeof(TaskDesc)) != hipSuccess) {
+      cap_med = 0;
+      return set_error(6, "allocation of %zu medium-task descriptors failed", c);
+    }
+    cap_med = c;
+  }
   if ((size_t)tot > cap_cand) {
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = (size_t)tot;
@@ -234,6 +249,12 @@ static void fill_block_map(uint32_t* out, int grid, int64_t gbeg, int64_t chunk,
   }
 }
 
+// Medium tasks: see MED_LEN.
+static inline bool is_medium(int64_t len, int64_t k, bool pred, const LaunchOpts& o) {
+  const int64_t m = k < len ? k : len;
+  return o.medium && !pred && len <= MED_LEN && m <= (int64_t)MED_CAP;
+}
+
 // Short tasks run whole in one block (k_small); small_cap 0 routes every
 // task through the streaming passes.
 static inline bool is_short(int64_t len, uint32_t small_cap) {
@@ -245,26 +266,31 @@ static inline bool is_short(int64_t len, uint32_t small_cap) {
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
                 hipStream_t s) {
   const uint32_t scap = eff_small_cap(f, o);
-  int64_t nl = 0, ns = 0, tot = 0, tot_pred = 0;
+  int64_t nl = 0, ns = 0, nm = 0, tot = 0, tot_med = 0;
   for (int64_t t = 0; t < n; ++t) {
-    if (is_short(tasks[t].len, scap)) {
+    const int64_t len = tasks[t].len, k = tasks[t].k;
+    if (is_short(len, scap)) {
       ++ns;
+    } else if (is_medium(len, k, o.predict && sample_groups(len, k) > 0, o)) {
+      ++nm;
+      tot_med += len;
     } else {
       ++nl;
-      tot += tasks[t].len;
-      if (o.predict && sample_groups(tasks[t].len, tasks[t].k)) tot_pred += tasks[t].len;
+      tot += len;
     }
   }
-  int rc = p.reserve(nl, ns, tot, s);
+  int rc = p.reserve(nl, ns, nm, tot + tot_med, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, base = 0, groups = 0, max_m = 0, runs = 0, msum = 0;
+  int64_t il = 0, is = 0, im = 0, base = 0, mbase = tot, groups = 0, max_m = 0, runs = 0, msum = 0;
+  int64_t max_m_med = 0;
   bool big = false;
-  // pass 0: predicted long tasks; pass 1: short tasks and the other long tasks
+  // pass 0: predicted long tasks; pass 1: the others
   for (int pass = 0; pass < 2; ++pass)
   for (int64_t t = 0; t < n; ++t) {
     const TaskSpec& ts = tasks[t];
     const bool shrt = is_short(ts.len, scap);
     const bool pred = o.predict && !shrt && sample_groups(ts.len, ts.k) > 0;
+    const bool med = !shrt && is_medium(ts.len, ts.k, pred, o);
     if ((pass == 0) != pred) continue;
     TaskDesc d;
     d.in = ts.in;
@@ -279,6 +305,12 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     if (shrt) {
       d.base = 0;
       p.h_small[is++] = d;
+    } else if (med) {
+      d.base = mbase;  // spill region behind the long tasks'
+      mbase += ts.len;
+      const int64_t m = ts.k < ts.len ? ts.k : ts.len;
+      if (m > max_m_med) max_m_med = m;
+      p.h_med[im++] = d;
     } else {
       d.base = base;
       p.h_bases[il] = base;
@@ -300,6 +332,8 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.h_cbases[il] = runs;
   p.h_mbases[il] = msum;
   p.has_big = big;
+  p.n_med = nm;
+  p.max_m_med = (uint32_t)max_m_med;
   // Launch shapes of the streaming kernels and their block -> task maps;
   // each long task learns how many pass-A blocks share it.
   p.grid_h = p.grid_s = 0;
@@ -335,7 +369,6 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.n_large = nl;
   p.n_small = ns;
   p.total = tot;
-  p.total_pred = tot_pred;
   p.total_groups = groups;
   p.max_m = (uint32_t)max_m;
   return 0;
@@ -353,6 +386,8 @@ int upload(Pipeline& p, hipStream_t s) {
   }
   if (p.n_small)
     RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
+  if (p.n_med)
+    RADIK_HIP(hipMemcpyAsync(p.d_med, p.h_med, p.n_med * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
   RADIK_HIP(hipEventRecord(p.upload_done, s));
   return 0;
 }
@@ -386,6 +421,25 @@ static int64_t stream_chunk(int64_t total, const DevFacts& f, const LaunchOpts& 
   return chunk;
 }
 
+// k_small and k_medium (independent of the long tasks).
+static int launch_short_and_medium(const Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s) {
+  if (p.n_small) {
+    const uint32_t scap = eff_small_cap(f, o);
+    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, s, p.d_small, scap);
+    RADIK_HIP(hipGetLastError());
+  }
+  if (p.n_med) {
+    const size_t lds = ((size_t)MED_CAP + std::min<uint32_t>(p.max_m_med, MED_CAP)) * 8;
+    const dim3 g((unsigned)p.n_med), b(BLOCK);
+    if (p.max_m_med <= (uint32_t)BLOCK) hipLaunchKernelGGL(k_medium<1>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
+    else if (p.max_m_med <= 2u * BLOCK) hipLaunchKernelGGL(k_medium<2>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
+    else if (p.max_m_med <= 4u * BLOCK) hipLaunchKernelGGL(k_medium<4>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
+    else hipLaunchKernelGGL(k_medium<8>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
+    RADIK_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
 int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm) {
   const bool timed = tm && tm->on;
   int phase = 0;
@@ -396,15 +450,14 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
   };
   int rc;
   if ((rc = mark())) return rc;  // start
-  // Short tasks are independent of the long ones: outside the phase timing
-  // they run on the side stream, overlapping the streaming passes.
-  const bool fork = !timed && p.n_small && p.n_large && p.side;
+  // Short and medium tasks are independent of the long ones: outside the
+  // phase timing they run on the side stream, overlapping the sampling and
+  // pass A.
+  const bool fork = !timed && (p.n_small || p.n_med) && p.n_large && p.side;
   if (fork) {
     RADIK_HIP(hipEventRecord(p.ev_fork, s));
     RADIK_HIP(hipStreamWaitEvent(p.side, p.ev_fork, 0));
-    const uint32_t scap = eff_small_cap(f, o);
-    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, p.side, p.d_small, scap);
-    RADIK_HIP(hipGetLastError());
+    if ((rc = launch_short_and_medium(p, f, o, p.side))) return rc;
     RADIK_HIP(hipEventRecord(p.ev_join, p.side));
   }
   if (p.n_large) {
@@ -517,10 +570,8 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
   if ((rc = mark())) return rc;  // end of sort
   if (fork) {
     RADIK_HIP(hipStreamWaitEvent(s, p.ev_join, 0));
-  } else if (p.n_small) {
-    const uint32_t scap = eff_small_cap(f, o);
-    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, s, p.d_small, scap);
-    RADIK_HIP(hipGetLastError());
+  } else if ((rc = launch_short_and_medium(p, f, o, s))) {
+    return rc;
   }
   if ((rc = mark())) return rc;  // end of small
   return 0;

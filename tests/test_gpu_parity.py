@@ -290,6 +290,36 @@ def test_prediction_miss_forces_reread():
     assert_same(run_ws(x, [n], k, O.LARGEST | O.ASCEND), oracle_topk(x, [n], k, O.LARGEST | O.ASCEND), "miss")
 
 
+def test_heavy_ties_at_the_threshold():
+    # Tens of thousands of exact ties in the threshold bin: the candidate set
+    # exceeds the finishing block's LDS, so the krem-th candidate is bracketed
+    # from a sample of the spill (ties resolve by lowest position).  Also a
+    # constant task and one batch mixing tie-heavy and plain tasks.
+    rng = np.random.default_rng(4242)
+    n = 1 << 20
+    x = rng.standard_normal(n).astype(np.float32)
+    x[rng.random(n) < 0.05] = np.float32(3.5)
+    for k in (64, 256, 3000):
+        for flags in (O.LARGEST, O.LARGEST | O.ASCEND, O.LARGEST | O.WITHSCALE):
+            say("heavy ties", k, flags)
+            assert_same(run_ws(x, [n], k, flags), oracle_topk(x, [n], k, flags), f"ties k{k} f{flags}")
+    c = np.full(1 << 18, 1.25, np.float32)
+    say("constant task")
+    assert_same(run_ws(c, [c.size], 100, O.LARGEST), oracle_topk(c, [c.size], 100, O.LARGEST), "constant")
+    lens = [1 << 19, 1 << 17, 1 << 18, 70000]
+    xs = rng.standard_normal(sum(lens)).astype(np.float32)
+    xs[: 1 << 19][rng.random(1 << 19) < 0.04] = np.float32(3.0)
+    xs[(1 << 19) + (1 << 17):][rng.random((1 << 18) + 70000) < 0.03] = np.float32(2.75)
+    say("tie mix")
+    assert_same(run_ws(xs, lens, 256, O.LARGEST), oracle_topk(xs, lens, 256, O.LARGEST), "tie mix")
+    # many tasks: the finishing blocks keep fewer candidates in LDS
+    lens = [1 << 16] * 300
+    xs = rng.standard_normal(sum(lens)).astype(np.float32)
+    xs[rng.random(xs.size) < 0.05] = np.float32(3.25)
+    say("tie batch")
+    assert_same(run_ws(xs, lens, 256, O.LARGEST), oracle_topk(xs, lens, 256, O.LARGEST), "tie batch")
+
+
 def test_large_k_multi_block_sort():
     # k above 4096 sorts in several blocks (runs + ranks); ties and specials
     rng = np.random.default_rng(16384)
