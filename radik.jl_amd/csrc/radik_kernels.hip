@@ -114,7 +114,8 @@ __device__ __forceinline__ uint32_t wave_append(bool p, uint64_t mask, uint32_t*
 // into the wave's stage; when the stage could overflow on the next append
 // (or at the end of a segment) the wave reserves space for all of them with
 // one global atomic and copies them out.  Entries are 64-bit; flush() hands
-// them to a writer (index in the destination, entry).
+// them to a writer (index in the destination, entry).  Between appends a
+// stage holds at most STAGE - 64 entries, so one append (<= 64) always fits.
 constexpr uint32_t STAGE = 512;  // entries per wave
 
 struct WaveStage {
@@ -397,11 +398,93 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
   auto write = [&](uint32_t slot, uint64_t v) {
     if (slot < slen) spill[slot] = v;
   };
-  stream_range<UNR>(d.in, s, e, [&](float x, uint32_t pos) {
+  auto one = [&](float x, uint32_t pos) {
     const uint32_t rk = rk_of<SCALE>(x, sc, xm);
     atomicAdd(&lh[rk >> 20], 1u);
     if (SPILL) stage_append(st, rk <= smax, ((uint64_t)rk << 32) | pos, ctr, write);
-  });
+  };
+  // Scalar head up to 16-byte alignment and tail, as in stream_range; the
+  // float4 body spills per iteration: the UNR*4 hits of each lane (a bit
+  // mask) enter the wave's stage together, placed by one ballot per element
+  // slot, so a rare spill costs a few scalar instructions per iteration.
+  const int64_t tid = threadIdx.x;
+  const float* __restrict__ p = d.in;
+  int64_t a = s;
+  const int64_t mis = (int64_t)(((uintptr_t)(p + a) >> 2) & 3u);
+  int64_t head_end = a + ((4 - mis) & 3);
+  if (head_end > e) head_end = e;
+  for (int64_t i = a + tid; i < head_end; i += BLOCK) one(p[i], (uint32_t)i);
+  a = head_end;
+  const int64_t nvec = (e - a) >> 2;
+  const f32x4* __restrict__ pv = reinterpret_cast<const f32x4*>(p + a);
+  int64_t v = tid;
+  for (; v + (int64_t)(UNR - 1) * BLOCK < nvec; v += (int64_t)UNR * BLOCK) {
+    f32x4 r[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) r[u] = __builtin_nontemporal_load(pv + v + (int64_t)u * BLOCK);
+    uint32_t hit = 0;
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t rk = rk_of<SCALE>(r[u][j], sc, xm);
+        atomicAdd(&lh[rk >> 20], 1u);
+        if (SPILL && rk <= smax) hit |= 1u << (4 * u + j);
+      }
+    }
+    if (SPILL && ballot(hit != 0)) {
+      uint64_t mk[4 * UNR];
+      uint32_t total = 0;
+#pragma unroll
+      for (int b = 0; b < 4 * UNR; ++b) {
+        mk[b] = ballot((hit >> b) & 1u);
+        total += (uint32_t)__builtin_popcountll(mk[b]);
+      }
+      // The stage keeps stage_append's invariant: at most STAGE - 64 entries
+      // between appends.
+      lds_wave_sync();
+      uint32_t n0 = *st.n;
+      if (n0 + total > STAGE - 64) {
+        stage_flush(st, ctr, write);
+        n0 = 0;
+      }
+      uint32_t off;
+      const bool direct = total > STAGE - 64;  // more than a stage holds: straight to the spill
+      if (direct) {
+        uint32_t base = 0;
+        if (lanes_below(ballot(true)) == 0) base = atomicAdd(ctr, total);
+        off = __builtin_amdgcn_readfirstlane(base);
+      } else {
+        off = n0;
+      }
+#pragma unroll
+      for (int b = 0; b < 4 * UNR; ++b) {
+        if ((hit >> b) & 1u) {
+          const int u = b >> 2, j = b & 3;
+          const uint32_t pos = (uint32_t)(a + ((v + (int64_t)u * BLOCK) << 2)) + (uint32_t)j;
+          const uint64_t ent = ((uint64_t)rk_of<SCALE>(r[u][j], sc, xm) << 32) | pos;
+          const uint32_t slot = off + lanes_below(mk[b]);
+          if (direct) write(slot, ent);
+          else st.buf[slot] = ent;
+        }
+        off += (uint32_t)__builtin_popcountll(mk[b]);
+      }
+      if (!direct) {
+        lds_wave_sync();
+        if (lanes_below(ballot(true)) == 0) *st.n = n0 + total;
+        lds_wave_sync();
+      }
+    }
+  }
+  for (; v < nvec; v += BLOCK) {
+    const f32x4 r = __builtin_nontemporal_load(pv + v);
+    const uint32_t q = (uint32_t)(a + (v << 2));
+    one(r.x, q);
+    one(r.y, q + 1);
+    one(r.z, q + 2);
+    one(r.w, q + 3);
+  }
+  for (int64_t i = a + (nvec << 2) + tid; i < e; i += BLOCK) one(p[i], (uint32_t)i);
   if (SPILL) stage_flush(st, ctr, write);
 }
 
