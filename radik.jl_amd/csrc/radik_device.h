@@ -57,6 +57,16 @@ static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
 
+// The task pointers (TaskDesc) address device global memory, but as fields
+// loaded from memory they reach the compiler as generic pointers, whose
+// (flat) accesses count against the LDS counter too and make every LDS wait
+// drain the outstanding loads.  Accesses through them go through glb().
+#define RADIK_AS1 __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ RADIK_AS1 T* glb(T* p) {
+  return (RADIK_AS1 T*)p;
+}
+
 // Order-preserving key of a float: unsigned order of key == numeric order,
 // with -0 < +0 and NaNs outside +-Inf (RadiK.jl src/utils.jl:29-33).
 __device__ __forceinline__ uint32_t fkey(float x) {
@@ -85,7 +95,7 @@ __device__ __forceinline__ uint32_t rank_key(float x, float s, uint32_t xm) {
 // (src/utils.jl:62-66); 0 when the task does not scale.
 __device__ __forceinline__ float task_scaler(const TaskDesc& d) {
   if (!(d.flags & F_WITHSCALE) || d.len <= 0) return 0.0f;
-  float s = d.in[0];
+  float s = glb(d.in)[0];
   return (isnan(s) || isinf(s)) ? 0.0f : s;
 }
 

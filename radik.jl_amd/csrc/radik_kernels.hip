@@ -51,17 +51,18 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // Visits elements [s, e) of task array p: scalar head up to 16-byte alignment,
 // UNR float4 loads in flight per thread over the body, scalar tail.  f(x, pos).
 template <int UNR, typename F>
-__device__ __forceinline__ void stream_range(const float* __restrict__ p, int64_t s, int64_t e, F&& f) {
+__device__ __forceinline__ void stream_range(const float* __restrict__ pp, int64_t s, int64_t e, F&& f) {
+  const RADIK_AS1 float* __restrict__ p = glb(pp);
   const int64_t tid = threadIdx.x;
   int64_t a = s;
   // elements until p + a is 16-byte aligned
-  int64_t mis = (int64_t)(((uintptr_t)(p + a) >> 2) & 3u);
+  int64_t mis = (int64_t)(((uintptr_t)(pp + a) >> 2) & 3u);
   int64_t head_end = a + ((4 - mis) & 3);
   if (head_end > e) head_end = e;
   for (int64_t i = a + tid; i < head_end; i += BLOCK) f(p[i], (uint32_t)i);
   a = head_end;
   const int64_t nvec = (e - a) >> 2;
-  const f32x4* __restrict__ pv = reinterpret_cast<const f32x4*>(p + a);
+  const RADIK_AS1 f32x4* __restrict__ pv = reinterpret_cast<const RADIK_AS1 f32x4*>(p + a);
   int64_t v = tid;
   for (; v + (int64_t)(UNR - 1) * BLOCK < nvec; v += (int64_t)UNR * BLOCK) {
     f32x4 r[UNR];
@@ -95,9 +96,9 @@ __device__ __forceinline__ uint32_t rk_of(float x, float sc, uint32_t xm) {
 
 // Stores a result (value, position) into output slot `slot` of task d.
 __device__ __forceinline__ void put_result(const TaskDesc& d, uint32_t slot, float v, uint32_t pos) {
-  d.val_out[slot] = v;
-  if (d.flags & F_IDX64) reinterpret_cast<int64_t*>(d.idx_out)[slot] = (int64_t)pos;
-  else reinterpret_cast<int32_t*>(d.idx_out)[slot] = (int32_t)pos;
+  glb(d.val_out)[slot] = v;
+  if (d.flags & F_IDX64) glb(reinterpret_cast<int64_t*>(d.idx_out))[slot] = (int64_t)pos;
+  else glb(reinterpret_cast<int32_t*>(d.idx_out))[slot] = (int32_t)pos;
 }
 
 // Wave-aggregated append: every lane with `p` gets a distinct slot from
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
     for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
       const int64_t g = g0 + u;
       const int64_t i = (g - tb) * d.len / G + threadIdx.x;
-      xv[u] = (g < se && i < d.len) ? d.in[i] : 0.0f;
+      xv[u] = (g < se && i < d.len) ? glb(d.in)[i] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
@@ -396,7 +397,7 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
                                                uint64_t* __restrict__ spill, const WaveStage& st) {
   const uint32_t slen = (uint32_t)d.len;
   auto write = [&](uint32_t slot, uint64_t v) {
-    if (slot < slen) spill[slot] = v;
+    if (slot < slen) glb(spill)[slot] = v;
   };
   auto one = [&](float x, uint32_t pos) {
     const uint32_t rk = rk_of<SCALE>(x, sc, xm);
@@ -408,15 +409,15 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
   // mask) enter the wave's stage together, placed by one ballot per element
   // slot, so a rare spill costs a few scalar instructions per iteration.
   const int64_t tid = threadIdx.x;
-  const float* __restrict__ p = d.in;
+  const RADIK_AS1 float* __restrict__ p = glb(d.in);
   int64_t a = s;
-  const int64_t mis = (int64_t)(((uintptr_t)(p + a) >> 2) & 3u);
+  const int64_t mis = (int64_t)(((uintptr_t)(d.in + a) >> 2) & 3u);
   int64_t head_end = a + ((4 - mis) & 3);
   if (head_end > e) head_end = e;
   for (int64_t i = a + tid; i < head_end; i += BLOCK) one(p[i], (uint32_t)i);
   a = head_end;
   const int64_t nvec = (e - a) >> 2;
-  const f32x4* __restrict__ pv = reinterpret_cast<const f32x4*>(p + a);
+  const RADIK_AS1 f32x4* __restrict__ pv = reinterpret_cast<const RADIK_AS1 f32x4*>(p + a);
   int64_t v = tid;
   for (; v + (int64_t)(UNR - 1) * BLOCK < nvec; v += (int64_t)UNR * BLOCK) {
     f32x4 r[UNR];
@@ -786,7 +787,7 @@ __device__ __forceinline__ void block_and_or(uint64_t a, uint64_t o, unsigned lo
 __device__ __forceinline__ float spilled_value(const TaskDesc& d, uint64_t x, uint32_t& pos) {
   pos = (uint32_t)x;
   if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
-  if (d.flags & F_WITHSCALE) return d.in[pos];
+  if (d.flags & F_WITHSCALE) return glb(d.in)[pos];
   const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
   return fkey_inv((uint32_t)(x >> 32) ^ xm);
 }
@@ -849,13 +850,13 @@ __device__ __forceinline__ void write_sorted(const TaskDesc& d, Src s, uint32_t 
       uint32_t pos = (uint32_t)x;
       if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
       v = fkey_inv((uint32_t)(x >> 32) ^ am);
-      if (withidx) id = i64 ? reinterpret_cast<const int64_t*>(d.idx_in)[pos]
-                            : (int64_t)reinterpret_cast<const int32_t*>(d.idx_in)[pos];
+      if (withidx) id = i64 ? glb(reinterpret_cast<const int64_t*>(d.idx_in))[pos]
+                            : (int64_t)glb(reinterpret_cast<const int32_t*>(d.idx_in))[pos];
       else id = (int64_t)pos + 1;
     }
-    d.val_out[i] = v;
-    if (i64) reinterpret_cast<int64_t*>(d.idx_out)[i] = id;
-    else reinterpret_cast<int32_t*>(d.idx_out)[i] = (int32_t)id;
+    glb(d.val_out)[i] = v;
+    if (i64) glb(reinterpret_cast<int64_t*>(d.idx_out))[i] = id;
+    else glb(reinterpret_cast<int32_t*>(d.idx_out))[i] = (int32_t)id;
   }
 }
 
@@ -932,14 +933,14 @@ __device__ __forceinline__ void write_slot(const TaskDesc& d, uint32_t i, uint64
     if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
     v = fkey_inv((uint32_t)(x >> 32) ^ am);
     if (d.flags & F_WITHIDXIN)
-      id = i64 ? reinterpret_cast<const int64_t*>(d.idx_in)[pos]
-               : (int64_t)reinterpret_cast<const int32_t*>(d.idx_in)[pos];
+      id = i64 ? glb(reinterpret_cast<const int64_t*>(d.idx_in))[pos]
+               : (int64_t)glb(reinterpret_cast<const int32_t*>(d.idx_in))[pos];
     else
       id = (int64_t)pos + 1;
   }
-  d.val_out[i] = v;
-  if (i64) reinterpret_cast<int64_t*>(d.idx_out)[i] = id;
-  else reinterpret_cast<int32_t*>(d.idx_out)[i] = (int32_t)id;
+  glb(d.val_out)[i] = v;
+  if (i64) glb(reinterpret_cast<int64_t*>(d.idx_out))[i] = id;
+  else glb(reinterpret_cast<int32_t*>(d.idx_out))[i] = (int32_t)id;
 }
 
 template <int E>
@@ -951,9 +952,9 @@ __device__ __forceinline__ void sort_task_regs(const TaskDesc& d, uint32_t m, ui
   for (int l = 0; l < E; ++l) {
     const uint32_t i = t * E + l;
     if (i < m) {
-      const uint32_t pos = i64 ? (uint32_t)reinterpret_cast<const int64_t*>(d.idx_out)[i]
-                               : (uint32_t)reinterpret_cast<const int32_t*>(d.idx_out)[i];
-      a[l] = out_composite(d, d.val_out[i], pos);
+      const uint32_t pos = i64 ? (uint32_t)glb(reinterpret_cast<const int64_t*>(d.idx_out))[i]
+                               : (uint32_t)glb(reinterpret_cast<const int32_t*>(d.idx_out))[i];
+      a[l] = out_composite(d, glb(d.val_out)[i], pos);
     } else {
       a[l] = ~0ull;
     }
@@ -1054,9 +1055,9 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
   } else {
     uint64_t* __restrict__ g = cand + d.base;
     for (uint32_t i = threadIdx.x; i < m; i += BLOCK) {
-      const uint32_t pos = i64 ? (uint32_t)reinterpret_cast<const int64_t*>(d.idx_out)[i]
-                               : (uint32_t)reinterpret_cast<const int32_t*>(d.idx_out)[i];
-      g[i] = out_composite(d, d.val_out[i], pos);
+      const uint32_t pos = i64 ? (uint32_t)glb(reinterpret_cast<const int64_t*>(d.idx_out))[i]
+                               : (uint32_t)glb(reinterpret_cast<const int32_t*>(d.idx_out))[i];
+      g[i] = out_composite(d, glb(d.val_out)[i], pos);
     }
     __threadfence_block();
     __syncthreads();
@@ -1093,9 +1094,9 @@ __global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restric
   for (int l = 0; l < 8; ++l) {
     const uint32_t i = j0 + threadIdx.x * 8 + l;
     if (i < m) {
-      const uint32_t pos = i64 ? (uint32_t)reinterpret_cast<const int64_t*>(d.idx_out)[i]
-                               : (uint32_t)reinterpret_cast<const int32_t*>(d.idx_out)[i];
-      a[l] = out_composite(d, d.val_out[i], pos);
+      const uint32_t pos = i64 ? (uint32_t)glb(reinterpret_cast<const int64_t*>(d.idx_out))[i]
+                               : (uint32_t)glb(reinterpret_cast<const int32_t*>(d.idx_out))[i];
+      a[l] = out_composite(d, glb(d.val_out)[i], pos);
     } else {
       a[l] = ~0ull;
     }
@@ -1262,7 +1263,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         const uint32_t slot = wave_append(cd, cm, &cctr);
         if (cd && slot < ncand) {
           if (fin) v[nwin + slot] = x;
-          else cdst[slot] = x;
+          else glb(cdst)[slot] = x;  // (kept apart from the LDS store: no flat access)
         }
       }
     }
@@ -1556,7 +1557,7 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
 #pragma unroll
     for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
       const uint32_t i = b + (uint32_t)u * BLOCK;
-      xs[u] = (i < n) ? d.in[i] : 0.0f;
+      xs[u] = (i < n) ? glb(d.in)[i] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
@@ -1584,7 +1585,7 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
         const uint32_t slot = wave_append(take, tm, &ectr);
         if (take && slot < m) {
           const uint32_t pos = (uint32_t)x;
-          const float v = scale ? d.in[pos] : fkey_inv((uint32_t)(x >> 32) ^ xm);
+          const float v = scale ? glb(d.in)[pos] : fkey_inv((uint32_t)(x >> 32) ^ xm);
           sel[slot] = out_composite(d, v, pos);
         }
       }
@@ -1617,7 +1618,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag_scan(const uint32_t* __restrict_
 __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst,
                                                      int64_t n16) {
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n16; i += (int64_t)gridDim.x * BLOCK)
-    dst[i] = __builtin_nontemporal_load(src + i);
+    glb(dst)[i] = __builtin_nontemporal_load(glb(src) + i);
 }
 
 // Explicit instantiations used by the host.
