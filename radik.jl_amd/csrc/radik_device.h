@@ -50,8 +50,9 @@ struct TaskCtl {
   uint32_t b_pred;    // predicted bound: pass A spills first digits <= b_pred (NO_BIN: no spill)
   uint32_t spill_all; // 1: the spill holds winners and candidates; 0: pass B re-reads the task
   uint32_t fin;       // 1: winners + candidates fit k_finish's LDS (nwin + ncand <= its capacity)
-  uint32_t done;      // pass-A blocks of the task that finished (the last one selects)
-  uint32_t pad[4];
+  uint32_t done;      // 1: the first digit is selected
+  uint32_t list_ctr;  // candidates listed by k_split
+  uint32_t pad[3];
 };
 static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 

@@ -71,6 +71,7 @@ struct Pipeline {
   int64_t* h_cbases = nullptr;  // sorted-run prefix sums (multi-block sort)
   int64_t* h_mbases = nullptr;  // result-count prefix sums (multi-block sort)
   uint32_t* h_bmap = nullptr;   // first task of each block of the streaming launches
+  uint2* h_xmap = nullptr;      // k_split: task / block in task / blocks of task, per block
   TaskDesc* h_small = nullptr;
   TaskDesc* h_med = nullptr;
   // device
@@ -80,13 +81,15 @@ struct Pipeline {
   int64_t* d_cbases = nullptr;
   int64_t* d_mbases = nullptr;
   uint32_t* d_bmap = nullptr;
+  uint2* d_xmap = nullptr;
   TaskDesc* d_small = nullptr;
   TaskDesc* d_med = nullptr;
   TaskCtl* d_ctl = nullptr;
   uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
   uint32_t* d_shist = nullptr;  // sample histograms, zero between launches
   uint64_t* d_cand = nullptr;   // spill regions
-  size_t cap_large = 0, cap_small = 0, cap_med = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0;
+  size_t cap_large = 0, cap_small = 0, cap_med = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0, cap_xmap = 0;
+  int64_t n_split = 0;  // k_split blocks (tasks longer than BIG_LEN)
   // Streaming launch shapes fixed at staging (passes A and B share one
   // chunking; sampling has its own): chunk, grid, offset of the block map.
   int64_t chunk_h = 0;

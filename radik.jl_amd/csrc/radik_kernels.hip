@@ -1202,10 +1202,19 @@ struct FinShared {
   TaskCtl ctl;
 };
 
+// Whether k_split left a spilled long task's winners and candidates listed
+// behind its spill: fin tasks winners then candidates, others (winners went
+// to the output slots) only candidates.
+__device__ __forceinline__ bool split_fits(const TaskCtl& c, int64_t len) {
+  const uint64_t n = c.cand_ctr, need = c.fin ? (uint64_t)c.nwin + c.ncand : (uint64_t)c.ncand;
+  return c.spill_all && c.m > 0 && n <= (uint64_t)len && n + need <= (uint64_t)len;
+}
+
 // Everything after the first-digit selection c of task d; see k_finish.
+// split: k_split ran for the tasks longer than big_len.
 template <int EMAX>
 __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c, uint64_t* __restrict__ cand,
-                                            uint64_t* v, uint32_t cap, int64_t big_len, int sort_here,
+                                            uint64_t* v, uint32_t cap, int64_t big_len, int sort_here, bool split,
                                             FinShared& S) {
   uint32_t* lh8 = S.lh8;
   uint32_t* wsum = S.wsum;
@@ -1226,9 +1235,22 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   const uint32_t len = (uint32_t)d.len;
   uint64_t* __restrict__ region = cand + d.base;
   const uint32_t n_sp = c.cand_ctr < len ? c.cand_ctr : len;  // spill, or pass B's candidates
-  const bool reread_done = !c.spill_all && d.len > big_len;    // k_compact re-read the task
-  const bool behind = c.spill_all && (uint64_t)n_sp + ncand <= (uint64_t)len;
-  uint64_t* __restrict__ cdst = c.spill_all ? region + n_sp : region;
+  const bool big = d.len > big_len;
+  // Where the winners and candidates are:
+  //   listed  - pass B over several blocks (a long task re-read) or k_split
+  //             left them listed: fin tasks winners then candidates, others
+  //             only candidates (the winners are in the output slots);
+  //   spilled - among the spill of pass A (winners, candidates, others);
+  //   neither - the task is re-read here.
+  const bool reread_done = !c.spill_all && big;
+  const bool listed = reread_done || (split && big && split_fits(c, d.len));
+  const uint64_t* __restrict__ lst = region + (reread_done ? 0u : n_sp);
+  const uint32_t nlst = fin ? nwin + ncand : (reread_done ? n_sp : ncand);
+  const bool spilled = c.spill_all && !listed;
+  // Candidates of non-fin tasks go to LDS when they fit, else (a re-read
+  // here) to the start of the spill region.
+  const uint32_t ncl = listed ? nlst : ncand;  // candidates of a non-fin task
+  const bool cand_lds = fin || ncl <= cap;
   if (threadIdx.x == 0) {
     wctr = 0;
     cctr = 0;
@@ -1259,31 +1281,31 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         a &= x;
         o |= x;
       }
-      if (fin || behind || !c.spill_all) {
-        const uint32_t slot = wave_append(cd, cm, &cctr);
-        if (cd && slot < ncand) {
-          if (fin) v[nwin + slot] = x;
-          else glb(cdst)[slot] = x;  // (kept apart from the LDS store: no flat access)
-        }
+      const uint32_t slot = wave_append(cd, cm, &cctr);
+      if (cd && slot < ncand) {
+        if (cand_lds) v[(fin ? nwin : 0u) + slot] = x;
+        else glb(region)[slot] = x;  // (kept apart from the LDS store: no flat access)
       }
     }
   };
-  // Many candidates of a spilled task: bound the krem-th smallest between two
-  // composites L <= U read off a sample of the spill (as pass A bounds the
-  // first digit), then one pass emits the winners and the candidates below
-  // L and keeps those in [L, U] in LDS; checked, else the exact digits below.
-  const bool sampled = !fin && c.spill_all && krem > 0 && ncand > cap;
+  // Many candidates (spilled or listed): bound the krem-th smallest between
+  // two composites L <= U read off a sample (as pass A bounds the first
+  // digit), then one pass emits the winners and the candidates below L and
+  // keeps those in [L, U] in LDS; checked, else the exact digits below.
+  const uint64_t* __restrict__ ssrc = spilled ? region : lst;
+  const uint32_t nss = spilled ? n_sp : nlst;
+  const bool sampled = !fin && (spilled || listed) && krem > 0 && ncl > cap;
   bool exact = true;
   if (sampled) {
     const uint32_t want = cap < 4096u ? cap : 4096u;
-    const uint32_t stride = n_sp / want > 1u ? n_sp / want : 1u;
-    const uint32_t ns = (n_sp + stride - 1) / stride;  // sampled spill entries j * stride
+    const uint32_t stride = nss / want > 1u ? nss / want : 1u;
+    const uint32_t ns = (nss + stride - 1) / stride;  // sampled entries j * stride
     for (uint32_t j0 = threadIdx.x; j0 < ns; j0 += BLOCK * LOADS_IN_FLIGHT) {
       uint64_t xs[LOADS_IN_FLIGHT];
 #pragma unroll
       for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
         const uint32_t j = j0 + (uint32_t)u * BLOCK;
-        xs[u] = (j < ns) ? region[(size_t)j * stride] : 0ull;
+        xs[u] = (j < ns) ? ssrc[(size_t)j * stride] : 0ull;
       }
 #pragma unroll
       for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
@@ -1303,7 +1325,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     __syncthreads();
     block_and_or(a, o, vb);
     const uint32_t cs = cctr < cap ? cctr : cap;
-    const float mu = (float)krem * (float)cs / (float)ncand;
+    const float mu = (float)krem * (float)cs / (float)ncl;
     const float dev = 4.0f * sqrtf(mu) + 4.0f;
     const int64_t qlo = (int64_t)floorf(mu - dev), qhi = (int64_t)ceilf(mu + dev);
     const uint64_t sv = (uint64_t)(vb[0] ^ vb[1]), ss = (uint64_t)vb[0];
@@ -1318,9 +1340,9 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     __syncthreads();
     a = ~0ull;
     o = 0ull;
-    for_each_entry(GlobalSrc{region}, n_sp, [&](uint64_t x) {
+    for_each_entry(GlobalSrc{ssrc}, nss, [&](uint64_t x) {
       const uint32_t dg = (uint32_t)(x >> 52);
-      const bool w = dg < bw;
+      const bool w = spilled && dg < bw;  // (a list holds candidates only)
       const bool lo = dg == bc && x < L;
       const bool mid = dg == bc && x >= L && x <= U;
       const uint64_t wm = ballot(w);
@@ -1378,17 +1400,39 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         }
       }
     } else {
-      // the sample misled: exact digits over the spill, every bit varying
+      // the sample misled: exact digits over the whole source, every bit varying
       a = 0ull;
       o = ~0ull;
       if (threadIdx.x == 0) ectr = 0;
     }
-  } else if (c.spill_all) {
+  } else if (listed) {
+    // fin: winners and candidates into LDS; else candidates (into LDS when
+    // they fit, else they stay listed)
+    for (uint32_t b0 = threadIdx.x; b0 < nlst; b0 += BLOCK * LOADS_IN_FLIGHT) {
+      uint64_t xs[LOADS_IN_FLIGHT];
+#pragma unroll
+      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+        const uint32_t i = b0 + (uint32_t)u * BLOCK;
+        xs[u] = (i < nlst) ? lst[i] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
+        const uint32_t i = b0 + (uint32_t)u * BLOCK;
+        if (i < nlst) {
+          if (cand_lds) v[i] = xs[u];
+          if (!fin || i >= nwin) {
+            a &= xs[u];
+            o |= xs[u];
+          }
+        }
+      }
+    }
+  } else if (spilled) {
     for_each_entry(GlobalSrc{region}, n_sp, [&](uint64_t x) {
       const uint32_t dg = (uint32_t)(x >> 52);
       take(x, dg < bw, dg == bc);
     });
-  } else if (!reread_done) {
+  } else {
     const float sc = task_scaler(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
     const uint32_t rmax = digit_bound(bc != NO_BIN ? bc : bw - 1);
@@ -1399,21 +1443,6 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         const uint32_t dg = rk >> 20;
         take(((uint64_t)rk << 32) | pos, hit && dg < bw, hit && dg == bc);
       }
-    });
-  } else if (fin) {  // k_compact left winners, then candidates, in the spill region
-    const uint32_t cnt = nwin + ncand;
-    for (uint32_t i = threadIdx.x; i < cnt; i += BLOCK) {
-      const uint64_t x = region[i];
-      v[i] = x;
-      if (i >= nwin) {
-        a &= x;
-        o |= x;
-      }
-    }
-  } else {  // k_compact emitted the winners and left the candidates in the spill region
-    for_each_entry(GlobalSrc{region}, n_sp, [&](uint64_t x) {
-      a &= x;
-      o |= x;
     });
   }
   __syncthreads();
@@ -1440,16 +1469,12 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     return;
   }
   if (krem > 0 && exact) {
-    // the candidates: compacted (cdst / the re-read's list) or among the spill
-    const bool mixed = c.spill_all && (!behind || sampled);
-    const uint64_t* __restrict__ src = mixed ? region : cdst;
-    const uint32_t nsrc = (mixed || reread_done) ? n_sp : ncand;
-    const bool in_lds = nsrc <= cap;
-    if (in_lds) {
-      for (uint32_t i = threadIdx.x; i < nsrc; i += BLOCK) v[i] = src[i];
-      __syncthreads();
-    }
-    if (krem < ncand) {
+    // the candidates: in LDS, listed, in the spill region after a re-read
+    // here, or (after a misleading sample) in its source
+    const uint64_t* __restrict__ src = sampled ? ssrc : (listed ? lst : region);
+    const uint32_t nsrc = sampled ? nss : ncl;
+    const bool in_lds = cand_lds && !sampled;
+    if (krem < ncl) {
       if (in_lds) s = select_digits<true>(LdsSrc{v}, nsrc, 52, P, krem, vary, same, lh8, wsum, res);
       else s = select_digits<false>(GlobalSrc{src}, nsrc, 52, P, krem, vary, same, (uint32_t*)v, wsum, res);
     }
@@ -1480,7 +1505,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
                                                   TaskCtl* __restrict__ ctl,
                                                   uint32_t* __restrict__ hist,
                                                   uint64_t* __restrict__ cand, uint32_t cap, uint32_t fin_cap,
-                                                  int64_t big_len, int sort_here) {
+                                                  int64_t big_len, int sort_here, int split) {
   extern __shared__ uint64_t v[];
   __shared__ FinShared S;
   const int t = blockIdx.x;
@@ -1494,7 +1519,81 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
     c = S.ctl;
     if (threadIdx.x == 0) ctl[t] = c;  // k_sort reads m
   }
-  finish_task<EMAX>(d, c, cand, v, cap, big_len, sort_here, S);
+  finish_task<EMAX>(d, c, cand, v, cap, big_len, sort_here, split != 0, S);
+}
+
+// ---------------------------------------------------------------------------
+// Split of the spills of long tasks (longer than big_len) over several blocks
+// ahead of k_finish, so one block does not read a long spill alone: winners
+// and candidates listed behind the spill (split_fits; fin tasks winners then
+// candidates, others candidates only with the winners in the output slots),
+// placed per tile of BLOCK * LOADS_IN_FLIGHT entries by one block scan and
+// one global atomic per list.  xmap: task, then block index in the task
+// (low 16 bits) and blocks of the task (high 16 bits), of each block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ tasks, TaskCtl* __restrict__ ctl,
+                                                 uint64_t* __restrict__ cand, const uint2* __restrict__ xmap) {
+  constexpr int U = LOADS_IN_FLIGHT;
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t slot0[2];
+  const uint2 tj = xmap[blockIdx.x];
+  const uint32_t t = tj.x, j = tj.y & 0xFFFFu, nb = tj.y >> 16;
+  const TaskCtl c = ctl[t];
+  const TaskDesc d = tasks[t];
+  if (!split_fits(c, d.len)) return;
+  const uint32_t n = c.cand_ctr;
+  const uint32_t per = (n + nb - 1) / nb;
+  const uint32_t s = min(n, j * per), e = min(n, s + per);
+  if (s >= e) return;
+  const bool fin = c.fin != 0;
+  const uint32_t bw = c.b_win, bc = c.b_cand, nwin = c.nwin, ncand = c.ncand;
+  uint64_t* __restrict__ spill = cand + d.base;
+  uint64_t* __restrict__ wdst = spill + n;
+  uint64_t* __restrict__ cdst = spill + n + (fin ? nwin : 0u);
+  for (uint32_t tile = s; tile < e; tile += (uint32_t)BLOCK * U) {
+    uint64_t xs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = tile + (uint32_t)u * BLOCK + threadIdx.x;
+      xs[u] = (i < e) ? spill[i] : 0ull;
+    }
+    uint32_t wmask = 0, cmask = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (tile + (uint32_t)u * BLOCK + threadIdx.x >= e) continue;
+      const uint32_t dg = (uint32_t)(xs[u] >> 52);
+      if (dg < bw) wmask |= 1u << u;
+      else if (dg == bc) cmask |= 1u << u;
+    }
+    const uint32_t mine = (uint32_t)__builtin_popcount(wmask) | ((uint32_t)__builtin_popcount(cmask) << 16);
+    const uint32_t incl = block_incl_scan(mine, wsum);
+    if (threadIdx.x == BLOCK - 1) {
+      const uint32_t tw = incl & 0xFFFFu, tc = incl >> 16;
+      slot0[0] = tw ? atomicAdd(&ctl[t].win_ctr, tw) : 0u;
+      slot0[1] = tc ? atomicAdd(&ctl[t].list_ctr, tc) : 0u;
+    }
+    __syncthreads();
+    const uint32_t excl = incl - mine;
+    uint32_t ws = slot0[0] + (excl & 0xFFFFu), cs = slot0[1] + (excl >> 16);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (wmask & (1u << u)) {
+        if (ws < nwin) {
+          if (fin) {
+            wdst[ws] = xs[u];
+          } else {
+            uint32_t pos;
+            const float v = spilled_value(d, xs[u], pos);
+            put_result(d, ws, v, pos);
+          }
+        }
+        ++ws;
+      } else if (cmask & (1u << u)) {
+        if (cs < ncand) cdst[cs] = xs[u];
+        ++cs;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1523,7 +1622,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
   select_first(d, h, NO_BIN, 0u, cap, &S.ctl, S.wsum, S.res);
   __syncthreads();  // the selection is published and the histogram read: v is free
   const TaskCtl c = S.ctl;
-  finish_task<EMAX>(d, c, cand, v, cap, INT64_MAX, 1, S);
+  finish_task<EMAX>(d, c, cand, v, cap, INT64_MAX, 1, false, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -1637,13 +1736,13 @@ template __global__ void k_medium<1>(const TaskDesc*, uint64_t*, uint32_t);
 template __global__ void k_medium<2>(const TaskDesc*, uint64_t*, uint32_t);
 template __global__ void k_medium<4>(const TaskDesc*, uint64_t*, uint32_t);
 template __global__ void k_medium<8>(const TaskDesc*, uint64_t*, uint32_t);
-template __global__ void k_finish<1>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
-template __global__ void k_finish<2>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
-template __global__ void k_finish<4>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
-template __global__ void k_finish<8>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
-template __global__ void k_finish<16>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
-template __global__ void k_finish<32>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
-template __global__ void k_finish<64>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int);
+template __global__ void k_finish<1>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
+template __global__ void k_finish<2>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
+template __global__ void k_finish<4>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
+template __global__ void k_finish<8>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
+template __global__ void k_finish<16>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
+template __global__ void k_finish<32>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
+template __global__ void k_finish<64>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
 template __global__ void k_sort<1>(const TaskDesc*, const TaskCtl*, uint64_t*);
 template __global__ void k_sort<2>(const TaskDesc*, const TaskCtl*, uint64_t*);
 template __global__ void k_sort<4>(const TaskDesc*, const TaskCtl*, uint64_t*);

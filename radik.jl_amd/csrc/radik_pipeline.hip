@@ -102,6 +102,8 @@ Pipeline::~Pipeline() {
   if (d_mbases) (void)hipFree(d_mbases);
   if (h_bmap) (void)hipHostFree(h_bmap);
   if (d_bmap) (void)hipFree(d_bmap);
+  if (h_xmap) (void)hipHostFree(h_xmap);
+  if (d_xmap) (void)hipFree(d_xmap);
   if (d_shist) (void)hipFree(d_shist);
   if (h_small) (void)hipHostFree(h_small);
   if (h_med) (void)hipHostFree(h_med);
@@ -363,6 +365,32 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     p.cap_bmap = c;
   }
   fill_block_map(p.h_bmap + p.off_h, p.grid_h, 0, p.chunk_h, p.h_bases, nl);
+  // k_split blocks of the tasks longer than BIG_LEN: one per 2^20 elements, up to 256
+  int64_t nsplit = 0;
+  for (int64_t t = 0; t < nl; ++t)
+    if (p.h_large[t].len > BIG_LEN) nsplit += std::min<int64_t>(256, p.h_large[t].len >> 20);
+  if (nsplit > 0 && (!p.h_xmap || (size_t)nsplit > p.cap_xmap)) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = 64;
+    while (c < (size_t)nsplit) c *= 2;
+    if (p.h_xmap) RADIK_HIP(hipHostFree(p.h_xmap));
+    if (p.d_xmap) RADIK_HIP(hipFree(p.d_xmap));
+    p.h_xmap = nullptr;
+    p.d_xmap = nullptr;
+    if (hipHostMalloc((void**)&p.h_xmap, c * sizeof(uint2), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&p.d_xmap, c * sizeof(uint2)) != hipSuccess) {
+      p.cap_xmap = 0;
+      return set_error(6, "allocation of a %zu-entry split map failed", c);
+    }
+    p.cap_xmap = c;
+  }
+  int64_t ix = 0;
+  for (int64_t t = 0; t < nl; ++t) {
+    if (p.h_large[t].len <= BIG_LEN) continue;
+    const uint32_t nb = (uint32_t)std::min<int64_t>(256, p.h_large[t].len >> 20);
+    for (uint32_t j = 0; j < nb; ++j) p.h_xmap[ix++] = make_uint2((uint32_t)t, j | (nb << 16));
+  }
+  p.n_split = nsplit;
   fill_block_map(p.h_bmap + p.off_s, p.grid_s, 0, SAMPLE_GROUPS_PER_BLOCK, p.h_sbases, nl);
   p.total_runs = runs;
   p.total_m = msum;
@@ -383,6 +411,8 @@ int upload(Pipeline& p, hipStream_t s) {
     RADIK_HIP(hipMemcpyAsync(p.d_mbases, p.h_mbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     if (p.n_bmap)
       RADIK_HIP(hipMemcpyAsync(p.d_bmap, p.h_bmap, p.n_bmap * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (p.n_split)
+      RADIK_HIP(hipMemcpyAsync(p.d_xmap, p.h_xmap, p.n_split * sizeof(uint2), hipMemcpyHostToDevice, s));
   }
   if (p.n_small)
     RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
@@ -519,20 +549,24 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
         default: launch_stream_kernels_compact<2>(p, s); break;
       }
       RADIK_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_split, dim3((unsigned)p.n_split), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
+                         p.d_xmap);
+      RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
     {
       const size_t lds = ((size_t)cap + sel) * 8;
       const dim3 g((unsigned)p.n_large), b(BLOCK);
       const int sh = sort_here ? 1 : 0;
+      const int sp = p.has_big ? 1 : 0;  // k_split listed the long tasks
       switch (emax) {
-        case 1: hipLaunchKernelGGL(k_finish<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
-        case 2: hipLaunchKernelGGL(k_finish<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
-        case 4: hipLaunchKernelGGL(k_finish<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
-        case 8: hipLaunchKernelGGL(k_finish<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
-        case 16: hipLaunchKernelGGL(k_finish<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
-        case 32: hipLaunchKernelGGL(k_finish<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
-        default: hipLaunchKernelGGL(k_finish<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh); break;
+        case 1: hipLaunchKernelGGL(k_finish<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
+        case 2: hipLaunchKernelGGL(k_finish<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
+        case 4: hipLaunchKernelGGL(k_finish<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
+        case 8: hipLaunchKernelGGL(k_finish<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
+        case 16: hipLaunchKernelGGL(k_finish<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
+        case 32: hipLaunchKernelGGL(k_finish<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
+        default: hipLaunchKernelGGL(k_finish<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
       }
       RADIK_HIP(hipGetLastError());
     }
