@@ -90,14 +90,18 @@ int radik_plan_set_graph(radik_plan *plan, int use_graph);
 /* Tuning knobs (results do not depend on them): vload_bytes 16/32/64 (bytes
  * in flight per lane per step of the streaming passes, 0 = keep), streaming
  * blocks per CU (0 = keep), longest task run whole in one block, candidates
- * held in LDS by the refinement (0 = device defaults), and whether long tasks
- * are sampled so one read of the input suffices. */
+ * held in LDS by the finishing kernel of launches with many tasks (0 = device
+ * defaults), and whether long tasks are sampled so one read of the input
+ * suffices. */
 int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu, int small_cap, int refine_cap,
                            int predict);
-/* Per-kernel timing: when enabled, execute runs eagerly and records HIP events
- * around each kernel on the plan's stream; radik_plan_timing returns the
- * accumulated milliseconds of the 8 phases (sample, predict, hist, select,
- * compact, refine, sort, small) and the number of executes timed. */
+/* Per-kernel timing: when enabled, execute runs eagerly (every kernel on the
+ * plan's stream) and records HIP events between the phases; radik_plan_timing
+ * returns the accumulated milliseconds of the 8 phases (sample, predict,
+ * hist = pass A with the selection of tasks inside one of its blocks, select
+ * and compact = the selection and re-read of tasks over 2^20 elements when
+ * present, finish, sort = only for results beyond the finishing kernel's
+ * LDS, small = short and medium tasks) and the number of executes timed. */
 int radik_plan_set_timing(radik_plan *plan, int enable);
 int radik_plan_timing(radik_plan *plan, double *ms8, int64_t *count);
 int radik_plan_destroy(radik_plan *plan);

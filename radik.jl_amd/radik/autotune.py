@@ -3,7 +3,7 @@
 Picks, per length bucket, the vector-load width (bytes in flight per lane per
 step of the streaming passes), the streaming blocks per CU, the longest task
 run whole in one block (k_small) and the candidates held in LDS by the
-refinement.  Results do not depend on these options, only speed does.
+finishing kernel.  Results do not depend on these options, only speed does.
 
 The search is coordinate descent: one knob at a time over its candidates,
 keeping the best, timed with HIP events over a few executions of the plan on

@@ -61,7 +61,7 @@ class Plan:
         ms = (ctypes.c_double * 8)()
         n = ctypes.c_int64()
         _lib.check(_lib.load().radik_plan_timing(self.handle, ms, ctypes.byref(n)))
-        names = ("sample", "predict", "hist", "select", "compact", "refine", "sort", "small")
+        names = ("sample", "predict", "hist", "select", "compact", "finish", "sort", "small")
         return {nm: ms[i] for i, nm in enumerate(names)}, n.value
 
     def __del__(self):
