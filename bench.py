@@ -56,8 +56,9 @@ def uneven_batch(T=4096, seed=4096):
 
 
 def dominant_roofline(plan, phase_ms, steps, N, out_pairs, idx_bytes):
-    """Roofline of the slower of the two streaming kernels (each reads the
-    input once: 4 B/element; compaction also writes the winner pairs)."""
+    """Roofline of the slower of the two streaming kernels: pass A (k_hist)
+    reads every element once (4 B/element); k_compact re-reads the long tasks
+    whose spill does not hold their selection and writes the winner pairs."""
     hist_ms = phase_ms["hist"] / steps
     comp_ms = phase_ms["compact"] / steps
     if comp_ms >= hist_ms:
@@ -67,6 +68,17 @@ def dominant_roofline(plan, phase_ms, steps, N, out_pairs, idx_bytes):
     achieved = byts / (t_ms * 1e-3)
     return name, {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic_from_profiles(name)}
+
+
+def path_fracs(ms, N, k, T, idx_bytes=4):
+    """Whole-path HBM fractions: under the two-read model of a radix select
+    with candidate spill (8 B/element + the k output pairs per task), and
+    with the one read of the input this pipeline makes (4 B/element + the
+    output pairs).  The second is the one that cannot exceed 1."""
+    out = (4 + idx_bytes) * k * T
+    t = ms * 1e-3
+    return {"path_hbm_frac": round((8.0 * N + out) / t / HBM_PEAK, 4),
+            "path_hbm_frac_one_read": round((4.0 * N + out) / t / HBM_PEAK, 4)}
 
 
 def traffic_from_profiles(kernel, workload="uneven"):
@@ -171,9 +183,8 @@ def secondary(radik, O, warmup, steps):
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / steps
         ph, cnt = phase_times(radik, p, vo, io, d, steps, di)
-        B = 8.0 * n + 8.0 * k
-        r = {"ms": round(ms, 4), "Gelem_s": round(n / (ms * 1e-3) / 1e9, 1),
-             "path_hbm_frac": round(B / (ms * 1e-3) / HBM_PEAK, 4)}
+        r = {"ms": round(ms, 4), "Gelem_s": round(n / (ms * 1e-3) / 1e9, 1)}
+        r.update(path_fracs(ms, n, k, 1))
         if ph:
             r["phase_ms"] = {kk: round(v / cnt, 4) for kk, v in ph.items()}
         res[f"k{k}"] = r
@@ -372,10 +383,9 @@ def main():
 
     phase, cnt = phase_times(radik, plan, vo, io, d_x, args.steps)
     kname, roof = dominant_roofline(plan, phase, cnt, N, k * T, 4)
-    path_frac = (8.0 * N + 8.0 * k * T) / (ms * 1e-3) / HBM_PEAK
 
     extra = {"phase_ms": {kk: round(v / cnt, 4) for kk, v in phase.items()}, "dominant_kernel": kname,
-             "path_hbm_frac": round(path_frac, 4), "wall_ms_per_step": round(wall_ms, 4),
+             **path_fracs(ms, N, k, T), "wall_ms_per_step": round(wall_ms, 4),
              "oracle_spot_check": check_ok}
 
     if not args.no_plain:
