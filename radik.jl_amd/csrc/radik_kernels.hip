@@ -369,6 +369,25 @@ __device__ __forceinline__ void take_hist(uint32_t* __restrict__ h, uint32_t (&c
   }
 }
 
+// take_hist in two steps: this thread's 16 bins, then their clearing.
+__device__ __forceinline__ void load_hist(const uint32_t* __restrict__ h, uint32_t (&c)[16]) {
+  const uint4* __restrict__ h4 = reinterpret_cast<const uint4*>(h) + threadIdx.x * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint4 v = h4[q];
+    c[4 * q + 0] = v.x;
+    c[4 * q + 1] = v.y;
+    c[4 * q + 2] = v.z;
+    c[4 * q + 3] = v.w;
+  }
+}
+
+__device__ __forceinline__ void clear_hist(uint32_t* __restrict__ h) {
+  uint4* __restrict__ h4 = reinterpret_cast<uint4*>(h) + threadIdx.x * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) h4[q] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // Standalone selection, one block per task, of the tasks pass A did not
 // select (those spread over several of its blocks): reads and clears the
 // task's histogram.
@@ -1174,22 +1193,71 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict_
 // ---------------------------------------------------------------------------
 constexpr int FIN_UNR = 8;  // float4 loads in flight per thread when re-reading a task
 
-// The q-th smallest (1 <= q <= n) of n distinct composites v[0, n) in LDS
-// that share the first digit `top`: the largest of the q smallest, which
-// the narrow digits isolate exactly.
-__device__ __forceinline__ uint64_t kth_smallest(const uint64_t* v, uint32_t n, uint32_t q, uint32_t top,
-                                                 uint64_t vary, uint64_t same, uint32_t* lh8, uint32_t* wsum,
-                                                 uint32_t* res, unsigned long long* mx) {
-  uint64_t P = top;
-  const int s = select_digits<true>(LdsSrc{v}, n, 52, P, q, vary, same, lh8, wsum, res);
-  if (threadIdx.x == 0) *mx = 0ull;
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
-    if ((v[i] >> s) <= P) atomicMax(mx, (unsigned long long)v[i]);
-  __syncthreads();
-  const uint64_t r = *mx;
-  __syncthreads();
-  return r;
+// Bounds L <= U with L at most the ql-th and U at least the qh-th smallest
+// (1 <= ql <= qh <= n) of n distinct composites v[0, n) in LDS that share the
+// first digit `top`, found together over the narrow digits: one histogram
+// per digit while both lie under one prefix, one each after they part.  The
+// lower side stops at a bin whose smallest member is its quantile (L = the
+// lowest composite of that bin), the upper one at a bin whose largest member
+// is (U = the highest composite of that bin), so no sample entry lies in
+// [L, U] beyond the ql-th .. qh-th smallest.
+__device__ __forceinline__ void sample_bounds(const uint64_t* v, uint32_t n, uint32_t ql, uint32_t qh, uint32_t top,
+                                              uint64_t vary, uint64_t same, uint32_t* lh8, uint32_t* wsum,
+                                              uint32_t* res, uint64_t& L, uint64_t& U) {
+  constexpr int shifts[7] = {44, 36, 32, 24, 16, 8, 0};
+  constexpr int widths[7] = {8, 8, 4, 8, 8, 8, 8};
+  uint64_t Pl = top, Ph = top;
+  int sl = 52, sh = 52;
+  bool dl = false, dh = false;
+  for (int q = 0; q < 7 && !(dl && dh); ++q) {
+    const int s = shifts[q], w = widths[q];
+    const uint32_t nb = 1u << w;
+    const int hs = s + w;
+    if (((vary >> s) & (nb - 1)) == 0) {  // one bin
+      const uint64_t fix = (same >> s) & (nb - 1);
+      if (!dl) {
+        Pl = (Pl << w) | fix;
+        sl = s;
+      }
+      if (!dh) {
+        Ph = (Ph << w) | fix;
+        sh = s;
+      }
+      continue;
+    }
+    const bool shared = !dl && !dh && Pl == Ph;
+    uint32_t c[1] = {0u};
+    for (int side = 0; side < 2; ++side) {
+      if (side == 0 ? dl : dh) continue;
+      const uint64_t P = side == 0 ? Pl : Ph;
+      if (!(shared && side == 1)) {  // (the shared histogram serves both)
+        for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) lh8[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+          const uint64_t x = v[i];
+          if ((x >> hs) == P) hist_add(lh8, (uint32_t)(x >> s) & (nb - 1));
+        }
+        __syncthreads();
+        c[0] = (threadIdx.x < nb) ? lh8[threadIdx.x] : 0u;
+      }
+      block_find_bin<1>(c, side == 0 ? ql : qh, wsum, res);
+      const uint32_t b = res[0], before = res[1], cnt = res[2];
+      __syncthreads();  // res and lh8 are rewritten next
+      if (side == 0) {
+        Pl = (Pl << w) | b;
+        sl = s;
+        ql -= before;
+        dl = ql == 1;
+      } else {
+        Ph = (Ph << w) | b;
+        sh = s;
+        qh -= before;
+        dh = qh == cnt;
+      }
+    }
+  }
+  L = Pl << sl;
+  U = ((Ph + 1ull) << sh) - 1ull;  // (wraps to ~0 at the top)
 }
 
 // LDS of a finishing block beside its dynamic entry array.
@@ -1198,7 +1266,7 @@ struct FinShared {
   uint32_t wsum[BLOCK / 64];
   uint32_t res[3];
   uint32_t wctr, cctr, ectr;
-  unsigned long long vb[2], mx;
+  unsigned long long vb[2];
   TaskCtl ctl;
 };
 
@@ -1223,7 +1291,6 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   uint32_t& cctr = S.cctr;
   uint32_t& ectr = S.ectr;
   unsigned long long* vb = S.vb;
-  unsigned long long& mx = S.mx;
   const uint32_t m = c.m;
   if (m == 0) {
     if (sort_here)
@@ -1297,9 +1364,12 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   const bool sampled = !fin && (spilled || listed) && krem > 0 && ncl > cap;
   bool exact = true;
   if (sampled) {
+    // every stride-th entry, at most `want` of them (all kept: a sample cut
+    // short would cover only the source's first part, whose positions need
+    // not be typical)
     const uint32_t want = cap < 4096u ? cap : 4096u;
-    const uint32_t stride = nss / want > 1u ? nss / want : 1u;
-    const uint32_t ns = (nss + stride - 1) / stride;  // sampled entries j * stride
+    const uint32_t stride = (nss + want - 1) / want;
+    const uint32_t ns = (nss + stride - 1) / stride;  // sampled entries j * stride, ns <= want
     for (uint32_t j0 = threadIdx.x; j0 < ns; j0 += BLOCK * LOADS_IN_FLIGHT) {
       uint64_t xs[LOADS_IN_FLIGHT];
 #pragma unroll
@@ -1329,9 +1399,14 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     const float dev = 4.0f * sqrtf(mu) + 4.0f;
     const int64_t qlo = (int64_t)floorf(mu - dev), qhi = (int64_t)ceilf(mu + dev);
     const uint64_t sv = (uint64_t)(vb[0] ^ vb[1]), ss = (uint64_t)vb[0];
-    const uint64_t L = (cs > 0 && qlo >= 1) ? kth_smallest(v, cs, (uint32_t)qlo, bc, sv, ss, lh8, wsum, res, &mx) : 0ull;
-    const uint64_t U = (cs > 0 && qhi <= (int64_t)cs) ? kth_smallest(v, cs, (uint32_t)qhi, bc, sv, ss, lh8, wsum, res, &mx)
-                                                      : ~0ull;
+    uint64_t L = 0ull, U = ~0ull;
+    if (cs > 0) {
+      const uint32_t ql = qlo < 1 ? 1u : (qlo > (int64_t)cs ? cs : (uint32_t)qlo);
+      const uint32_t qh = qhi > (int64_t)cs ? cs : (qhi < 1 ? 1u : (uint32_t)qhi);
+      sample_bounds(v, cs, ql, qh, bc, sv, ss, lh8, wsum, res, L, U);
+      if (qlo < 1) L = 0ull;
+      if (qhi > (int64_t)cs) U = ~0ull;
+    }
     if (threadIdx.x == 0) {
       cctr = 0;
       vb[0] = ~0ull;
@@ -1511,9 +1586,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
   const int t = blockIdx.x;
   const TaskDesc d = tasks[t];
   TaskCtl c = ctl[t];
+  // The histogram is read before `done` is known (one latency instead of
+  // two); it is all zero for tasks pass A selected, and left so.
+  uint32_t* __restrict__ th = hist + (size_t)t * HIST_BINS;
+  uint32_t h[16];
+  load_hist(th, h);
   if (!c.done) {  // spread over several pass-A blocks: select from the global histogram
-    uint32_t h[16];
-    take_hist(hist + (size_t)t * HIST_BINS, h);
+    clear_hist(th);
     select_first(d, h, c.b_pred, c.cand_ctr, fin_cap, &S.ctl, S.wsum, S.res);
     __syncthreads();
     c = S.ctl;
