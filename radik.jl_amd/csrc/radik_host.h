@@ -32,6 +32,7 @@ struct DevFacts {
   uint32_t refine_cap = 4096;  // default: candidates kept in LDS by k_refine
   uint32_t small_cap_max = 2048, refine_cap_max = 4096;  // what the LDS budget allows
   uint32_t sort_cap = 4096;    // results sorted in LDS by k_sort
+  int hist_blocks_per_cu = 4;  // resident pass-A blocks per CU (occupancy of k_hist)
 };
 int dev_facts(DevFacts** out);
 
@@ -70,7 +71,8 @@ struct Pipeline {
   int64_t* h_sbases = nullptr;  // sample-group prefix sums of the long tasks
   int64_t* h_cbases = nullptr;  // sorted-run prefix sums (multi-block sort)
   int64_t* h_mbases = nullptr;  // result-count prefix sums (multi-block sort)
-  uint32_t* h_bmap = nullptr;   // first task of each block of the streaming launches
+  uint32_t* h_bmap = nullptr;   // first task of each chunk of the streaming launches
+  int64_t* h_cstart = nullptr;  // pass A's chunk table: nch + 1 starts in the element space
   uint2* h_xmap = nullptr;      // k_split: task / block in task / blocks of task, per block
   TaskDesc* h_small = nullptr;
   TaskDesc* h_med = nullptr;
@@ -81,6 +83,8 @@ struct Pipeline {
   int64_t* d_cbases = nullptr;
   int64_t* d_mbases = nullptr;
   uint32_t* d_bmap = nullptr;
+  int64_t* d_cstart = nullptr;
+  uint32_t* d_work = nullptr;   // pass A's chunk counter (zeroed by k_predict)
   uint2* d_xmap = nullptr;
   TaskDesc* d_small = nullptr;
   TaskDesc* d_med = nullptr;
@@ -88,11 +92,13 @@ struct Pipeline {
   uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
   uint32_t* d_shist = nullptr;  // sample histograms, zero between launches
   uint64_t* d_cand = nullptr;   // spill regions
-  size_t cap_large = 0, cap_small = 0, cap_med = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0, cap_xmap = 0;
+  size_t cap_large = 0, cap_small = 0, cap_med = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0, cap_xmap = 0,
+         cap_cstart = 0;
   int64_t n_split = 0;  // k_split blocks (tasks longer than BIG_LEN)
-  // Streaming launch shapes fixed at staging (passes A and B share one
-  // chunking; sampling has its own): chunk, grid, offset of the block map.
-  int64_t chunk_h = 0;
+  // Streaming launch shapes fixed at staging (passes A and B share one chunk
+  // table; sampling has equal chunks): pass A's persistent grid, the number
+  // of chunks, the sampling grid, offsets of their maps in the block map.
+  int64_t n_chunks = 0;
   int grid_h = 0, grid_s = 0;
   int64_t off_h = 0, off_s = 0, n_bmap = 0;
   // current content

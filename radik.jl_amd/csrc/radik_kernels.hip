@@ -28,9 +28,11 @@
 //   k_small    one block per short task: everything above in LDS, one launch.
 //
 // Streaming kernels (k_sample, k_hist, k_compact) split the launch's
-// concatenated element space into equal chunks, one per block, so uneven task
-// lengths balance at element granularity; a chunk that crosses task boundaries
-// flushes per task.
+// concatenated element space into chunks, so uneven task lengths balance at
+// element granularity; a chunk that crosses task boundaries flushes per task.
+// k_sample takes equal chunks, one per block; k_hist takes the chunks of a
+// host table, shrinking towards its end, in the order of a device counter;
+// k_compact takes the same table one chunk per block.
 
 #include "radik_device.h"
 
@@ -252,10 +254,11 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
 __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ sbases,
                                                    uint32_t* __restrict__ shist,
-                                                   TaskCtl* __restrict__ ctl) {
+                                                   TaskCtl* __restrict__ ctl, uint32_t* __restrict__ work) {
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
   const int t = blockIdx.x;
+  if (t == 0 && threadIdx.x == 0) *work = 0u;  // pass A's chunk counter
   const TaskDesc d = tasks[t];
   const int64_t G = sbases[t + 1] - sbases[t];
   const int64_t m = d.k < d.len ? (int64_t)d.k : d.len;
@@ -515,31 +518,26 @@ __device__ __forceinline__ uint32_t load_dev(const uint32_t* p) {
 }
 
 
-// SELECT: a task inside one block's chunk (nblk == 1) is selected from that
-// block's LDS histogram right after its segment, without a global histogram;
-// the others (and every task without SELECT) add their counts to the task's
-// global histogram, selected after the pass.
+// Pass A over one chunk [g0, g1) of the element space, task segment by task
+// segment (t: the task holding g0).  SELECT: a task inside the chunk
+// (nblk == 1) is selected from the block's LDS histogram right after its
+// segment, without a global histogram; the others (and every task without
+// SELECT) add their counts to the task's global histogram, selected after the
+// pass.  A task that runs on past the chunk keeps its counts in lh (pend =
+// the task, else -1) until the block's next chunk starts elsewhere, so one
+// long task is flushed about once per block, not once per chunk.  Ends
+// without a barrier.
 template <int UNR, bool SELECT>
-__global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tasks,
-                                                const int64_t* __restrict__ bases, int ntasks,
-                                                int64_t total, int64_t chunk,
-                                                uint32_t* __restrict__ hist,
-                                                TaskCtl* __restrict__ ctl,
-                                                uint64_t* __restrict__ cand,
-                                                const uint32_t* __restrict__ bmap, uint32_t fin_cap) {
-  __shared__ uint32_t lh[HIST_BINS];
-  __shared__ uint64_t stage_buf[BLOCK / 64][STAGE];
-  __shared__ uint32_t stage_n[BLOCK / 64];
-  __shared__ uint32_t wsum[BLOCK / 64];
-  __shared__ uint32_t res[3];
-  for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
-  if (threadIdx.x < BLOCK / 64) stage_n[threadIdx.x] = 0;
-  const WaveStage st{stage_buf[threadIdx.x >> 6], &stage_n[threadIdx.x >> 6]};
-  int64_t g0 = (int64_t)blockIdx.x * chunk;
-  const int64_t g1 = min(g0 + chunk, total);
-  if (g0 >= g1) return;
-  int t = (int)bmap[blockIdx.x];  // = find_task(bases, ntasks, g0), from the host
-  __syncthreads();
+__device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, const int64_t* __restrict__ bases,
+                                           int ntasks, int64_t g0, int64_t g1, int t, uint32_t* __restrict__ hist,
+                                           TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
+                                           uint32_t fin_cap, uint32_t* lh, const WaveStage& st, uint32_t* wsum,
+                                           uint32_t* res, int& pend) {
+  if (pend >= 0 && pend != t) {
+    flush_hist(lh, hist + (size_t)pend * HIST_BINS);
+    __syncthreads();
+  }
+  pend = -1;
   while (true) {
     const TaskDesc d = tasks[t];
     const int64_t tb = bases[t];
@@ -560,7 +558,9 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
       else pass_a_segment<UNR, false, true>(d, s, e, sc, xm, smax, lh, ctr, spill, st);
     }
     __syncthreads();
-    if (!SELECT) {
+    if (se >= g1 && se < tb + d.len) {
+      pend = t;  // continues past the chunk
+    } else if (!SELECT) {
       flush_hist(lh, hist + (size_t)t * HIST_BINS);
     } else if (d.nblk <= 1) {
       // the whole task was in this chunk: select from the LDS histogram
@@ -580,6 +580,46 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
     while (t + 1 < ntasks && bases[t + 1] <= g0) ++t;  // skip empty tasks
     __syncthreads();
   }
+}
+
+// Pass A, persistent: each block takes chunks of the host's chunk table
+// (cstart: nch + 1 starts; cmap: task holding each chunk's start) in the
+// order of a device counter (`work`, zeroed by k_predict), so a block that
+// starts late (its CU busy with other kernels) takes fewer chunks; the
+// table's chunks shrink towards its end, so the blocks finish together.
+template <int UNR, bool SELECT>
+__global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tasks,
+                                                const int64_t* __restrict__ bases, int ntasks,
+                                                const int64_t* __restrict__ cstart, int nch,
+                                                uint32_t* __restrict__ work,
+                                                uint32_t* __restrict__ hist,
+                                                TaskCtl* __restrict__ ctl,
+                                                uint64_t* __restrict__ cand,
+                                                const uint32_t* __restrict__ cmap, uint32_t fin_cap) {
+  __shared__ uint32_t lh[HIST_BINS];
+  __shared__ uint64_t stage_buf[BLOCK / 64][STAGE];
+  __shared__ uint32_t stage_n[BLOCK / 64];
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t res[3];
+  __shared__ int next_chunk;
+  for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
+  if (threadIdx.x < BLOCK / 64) stage_n[threadIdx.x] = 0;
+  if (threadIdx.x == 0) next_chunk = (int)atomicAdd(work, 1u);
+  const WaveStage st{stage_buf[threadIdx.x >> 6], &stage_n[threadIdx.x >> 6]};
+  __syncthreads();
+  int ch = next_chunk;
+  int pend = -1;
+  while (ch < nch) {
+    int ch_next = 0;  // fetched while this chunk streams
+    if (threadIdx.x == 0) ch_next = (int)atomicAdd(work, 1u);
+    hist_chunk<UNR, SELECT>(tasks, bases, ntasks, cstart[ch], cstart[ch + 1], (int)cmap[ch], hist, ctl, cand,
+                            fin_cap, lh, st, wsum, res, pend);
+    __syncthreads();  // lh and the stages are free; every thread has read next_chunk
+    if (threadIdx.x == 0) next_chunk = ch_next;
+    __syncthreads();
+    ch = next_chunk;
+  }
+  if (pend >= 0) flush_hist(lh, hist + (size_t)pend * HIST_BINS);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,10 +667,10 @@ __device__ __forceinline__ void pass_b_segment(const TaskDesc& d, int64_t s, int
 template <int UNR>
 __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ bases, int ntasks,
-                                                   int64_t g_begin, int64_t total, int64_t chunk,
+                                                   const int64_t* __restrict__ cstart,
                                                    TaskCtl* __restrict__ ctl,
                                                    uint64_t* __restrict__ cand,
-                                                   const uint32_t* __restrict__ bmap, int64_t min_len) {
+                                                   const uint32_t* __restrict__ cmap, int64_t min_len) {
   __shared__ uint64_t stage_w[BLOCK / 64][STAGE];
   __shared__ uint64_t stage_c[BLOCK / 64][STAGE];
   __shared__ uint32_t stage_n[2][BLOCK / 64];
@@ -638,10 +678,11 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
   const WaveStage sw{stage_w[threadIdx.x >> 6], &stage_n[0][threadIdx.x >> 6]};
   const WaveStage scd{stage_c[threadIdx.x >> 6], &stage_n[1][threadIdx.x >> 6]};
   __syncthreads();
-  int64_t g0 = g_begin + (int64_t)blockIdx.x * chunk;
-  const int64_t g1 = min(g0 + chunk, total);
+  // one chunk of pass A's table per block
+  int64_t g0 = cstart[blockIdx.x];
+  const int64_t g1 = cstart[blockIdx.x + 1];
   if (g0 >= g1) return;
-  int t = (int)bmap[blockIdx.x];  // = find_task(bases, ntasks, g0), from the host
+  int t = (int)cmap[blockIdx.x];  // = find_task(bases, ntasks, g0), from the host
   while (true) {
     const TaskDesc d = tasks[t];
     const int64_t tb = bases[t];
@@ -1800,13 +1841,13 @@ __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ s
 }
 
 // Explicit instantiations used by the host.
-#define RADIK_INST(U)                                                                                   \
-  template __global__ void k_hist<U, true>(const TaskDesc*, const int64_t*, int, int64_t, int64_t,      \
-                                           uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t);  \
-  template __global__ void k_hist<U, false>(const TaskDesc*, const int64_t*, int, int64_t, int64_t,     \
-                                            uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t); \
-  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, int64_t, int64_t, int64_t,   \
-                                        TaskCtl*, uint64_t*, const uint32_t*, int64_t);
+#define RADIK_INST(U)                                                                                      \
+  template __global__ void k_hist<U, true>(const TaskDesc*, const int64_t*, int, const int64_t*, int, uint32_t*, \
+                                           uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t);           \
+  template __global__ void k_hist<U, false>(const TaskDesc*, const int64_t*, int, const int64_t*, int, uint32_t*, \
+                                            uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t);          \
+  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, const int64_t*, TaskCtl*,         \
+                                        uint64_t*, const uint32_t*, int64_t);
 RADIK_INST(1)
 RADIK_INST(2)
 RADIK_INST(4)

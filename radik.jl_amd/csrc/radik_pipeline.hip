@@ -72,6 +72,13 @@ int dev_facts(DevFacts** out) {
       }
     }
     f.lds_per_block = budget;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_hist<4, true>, BLOCK, 0) != hipSuccess ||
+        occ < 1) {
+      (void)hipGetLastError();
+      occ = 4;
+    }
+    f.hist_blocks_per_cu = occ;
     const size_t stat = 20 * 1024;  // static LDS of k_small (histogram + scratch)
     f.refine_cap_max = (uint32_t)((budget - FINISH_STATIC_LDS) / 8);
     f.small_cap_max = (uint32_t)((budget - stat) / 16);
@@ -102,6 +109,9 @@ Pipeline::~Pipeline() {
   if (d_mbases) (void)hipFree(d_mbases);
   if (h_bmap) (void)hipHostFree(h_bmap);
   if (d_bmap) (void)hipFree(d_bmap);
+  if (h_cstart) (void)hipHostFree(h_cstart);
+  if (d_cstart) (void)hipFree(d_cstart);
+  if (d_work) (void)hipFree(d_work);
   if (h_xmap) (void)hipHostFree(h_xmap);
   if (d_xmap) (void)hipFree(d_xmap);
   if (d_shist) (void)hipFree(d_shist);
@@ -239,7 +249,35 @@ static int64_t sample_groups(int64_t len, int64_t k) {
   return g;
 }
 
-static int64_t stream_chunk(int64_t total, const DevFacts& f, const LaunchOpts& o, int* grid);
+// Pass A's chunk table over `total` elements: guided sizes, each about the
+// remaining elements / (spread * blocks), at least min_chunk and a multiple
+// of gran (a last piece under half of min_chunk joins its predecessor), so
+// the chunks taken last are small and the blocks finish together.  Writes
+// the nch + 1 starts when `out` is non-null; returns nch.
+static int64_t plan_chunks(int64_t total, int64_t blocks, int64_t spread, int64_t gran, int64_t min_chunk,
+                           int64_t* out) {
+  int64_t n = 0, g = 0;
+  while (g < total) {
+    if (out) out[n] = g;
+    const int64_t rem = total - g;
+    int64_t c = rem / (spread * blocks);
+    if (c < min_chunk) c = min_chunk;
+    c = (c + gran - 1) / gran * gran;
+    g = (rem - c < min_chunk / 2) ? total : g + c;
+    ++n;
+  }
+  if (out) out[n] = total;
+  return n;
+}
+
+// First task (largest t with pre[t] <= g) of each chunk start g = starts[c].
+static void fill_chunk_map(uint32_t* out, int64_t nch, const int64_t* starts, const int64_t* pre, int64_t nt) {
+  int64_t t = 0;
+  for (int64_t c = 0; c < nch; ++c) {
+    while (t + 1 < nt && pre[t + 1] <= starts[c]) ++t;
+    out[c] = (uint32_t)t;
+  }
+}
 
 // First task (largest t with pre[t] <= g0) of each block b, g0 = gbeg + b * chunk.
 static void fill_block_map(uint32_t* out, int grid, int64_t gbeg, int64_t chunk, const int64_t* pre, int64_t nt) {
@@ -336,18 +374,58 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.has_big = big;
   p.n_med = nm;
   p.max_m_med = (uint32_t)max_m_med;
-  // Launch shapes of the streaming kernels and their block -> task maps;
-  // each long task learns how many pass-A blocks share it.
+  // Launch shapes of the streaming kernels, pass A's chunk table and the
+  // chunk -> task maps; each long task learns how many chunks share it.
+  // Pass A runs min(blocks_per_cu, resident blocks per CU) persistent blocks
+  // per CU; its table's chunks shrink from remaining / (spread * blocks) down
+  // to min_chunk, spread = how far blocks_per_cu exceeds the resident count.
   p.grid_h = p.grid_s = 0;
-  p.chunk_h = nl ? stream_chunk(tot, f, o, &p.grid_h) : 0;
+  const int res_cu = std::max(1, f.hist_blocks_per_cu);
+  const int64_t slots = (int64_t)f.cus * std::max(1, std::min(o.blocks_per_cu, res_cu));
+  const int64_t spread = std::max(1, o.blocks_per_cu / res_cu);
+  const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
+  const int64_t min_chunk = 8 * gran;
+  const int64_t nch = nl ? plan_chunks(tot, slots, spread, gran, min_chunk, nullptr) : 0;
+  if (nch > 0 && (!p.h_cstart || (size_t)nch + 1 > p.cap_cstart)) {
+    RADIK_HIP(hipStreamSynchronize(s));
+    size_t c = 64;
+    while (c < (size_t)nch + 1) c *= 2;
+    if (p.h_cstart) RADIK_HIP(hipHostFree(p.h_cstart));
+    if (p.d_cstart) RADIK_HIP(hipFree(p.d_cstart));
+    p.h_cstart = nullptr;
+    p.d_cstart = nullptr;
+    if (hipHostMalloc((void**)&p.h_cstart, c * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&p.d_cstart, c * sizeof(int64_t)) != hipSuccess) {
+      p.cap_cstart = 0;
+      return set_error(6, "allocation of a %zu-entry chunk table failed", c);
+    }
+    p.cap_cstart = c;
+  }
+  if (nl > 0 && !p.d_work && hipMalloc((void**)&p.d_work, 256) != hipSuccess) {
+    p.d_work = nullptr;
+    return set_error(6, "allocation of the chunk counter failed");
+  }
+  if (nch > 0) plan_chunks(tot, slots, spread, gran, min_chunk, p.h_cstart);
+  p.n_chunks = nch;
+  p.grid_h = (int)std::min<int64_t>(nch, slots);
   p.grid_s = groups > 0 ? (int)((groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK) : 0;
-  for (int64_t t = 0; t < nl; ++t) {
-    const int64_t b0 = p.h_bases[t], len = p.h_large[t].len;
-    p.h_large[t].nblk = len > 0 ? (int32_t)((b0 + len - 1) / p.chunk_h - b0 / p.chunk_h + 1) : 0;
-    p.h_large[t].spare = 0;
+  {
+    int64_t c = 0;
+    for (int64_t t = 0; t < nl; ++t) {
+      const int64_t b0 = p.h_bases[t], len = p.h_large[t].len;
+      p.h_large[t].spare = 0;
+      if (len <= 0) {
+        p.h_large[t].nblk = 0;
+        continue;
+      }
+      while (c + 1 < nch && p.h_cstart[c + 1] <= b0) ++c;  // chunk holding the first element
+      int64_t c2 = c;
+      while (c2 + 1 < nch && p.h_cstart[c2 + 1] <= b0 + len - 1) ++c2;  // ... and the last
+      p.h_large[t].nblk = (int32_t)(c2 - c + 1);
+    }
   }
   p.off_h = 0;
-  p.off_s = p.off_h + p.grid_h;
+  p.off_s = p.off_h + nch;
   p.n_bmap = p.off_s + p.grid_s;
   if (!p.h_bmap || (size_t)p.n_bmap > p.cap_bmap) {
     RADIK_HIP(hipStreamSynchronize(s));
@@ -364,7 +442,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     }
     p.cap_bmap = c;
   }
-  fill_block_map(p.h_bmap + p.off_h, p.grid_h, 0, p.chunk_h, p.h_bases, nl);
+  fill_chunk_map(p.h_bmap + p.off_h, nch, p.h_cstart, p.h_bases, nl);
   // k_split blocks of the tasks longer than BIG_LEN: one per 2^20 elements, up to 256
   int64_t nsplit = 0;
   for (int64_t t = 0; t < nl; ++t)
@@ -411,6 +489,8 @@ int upload(Pipeline& p, hipStream_t s) {
     RADIK_HIP(hipMemcpyAsync(p.d_mbases, p.h_mbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     if (p.n_bmap)
       RADIK_HIP(hipMemcpyAsync(p.d_bmap, p.h_bmap, p.n_bmap * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (p.n_chunks)
+      RADIK_HIP(hipMemcpyAsync(p.d_cstart, p.h_cstart, (p.n_chunks + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     if (p.n_split)
       RADIK_HIP(hipMemcpyAsync(p.d_xmap, p.h_xmap, p.n_split * sizeof(uint2), hipMemcpyHostToDevice, s));
   }
@@ -428,27 +508,19 @@ int upload(Pipeline& p, hipStream_t s) {
 // Pass A with the fused selection (fin_cap: see k_finish).
 template <int UNR>
 static void launch_stream_kernels_hist(const Pipeline& p, uint32_t fin_cap, hipStream_t s) {
+  if (p.grid_h <= 0) return;
   hipLaunchKernelGGL((k_hist<UNR, true>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                     (int)p.n_large, p.total, p.chunk_h, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
+                     (int)p.n_large, p.d_cstart, (int)p.n_chunks, p.d_work, p.d_hist, p.d_ctl, p.d_cand,
+                     p.d_bmap + p.off_h, fin_cap);
 }
 
-// Pass B over the whole element space (pass A's chunks); blocks skip tasks of
-// at most BIG_LEN elements and those with a usable spill.
+// Pass B over the whole element space (pass A's chunks, one per block);
+// blocks skip tasks of at most BIG_LEN elements and those with a usable spill.
 template <int UNR>
 static void launch_stream_kernels_compact(const Pipeline& p, hipStream_t s) {
-  hipLaunchKernelGGL(k_compact<UNR>, dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases, (int)p.n_large,
-                     (int64_t)0, p.total, p.chunk_h, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN);
-}
-
-// Equal chunks of `total` elements, about blocks_per_cu blocks per CU.
-static int64_t stream_chunk(int64_t total, const DevFacts& f, const LaunchOpts& o, int* grid) {
-  const int64_t want_blocks = (int64_t)f.cus * o.blocks_per_cu;
-  int64_t chunk = (total + want_blocks - 1) / want_blocks;
-  const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
-  if (chunk < 4 * gran) chunk = 4 * gran;
-  chunk = (chunk + gran - 1) / gran * gran;
-  *grid = (int)std::max<int64_t>(1, (total + chunk - 1) / chunk);
-  return chunk;
+  if (p.n_chunks <= 0) return;
+  hipLaunchKernelGGL(k_compact<UNR>, dim3((unsigned)p.n_chunks), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
+                     (int)p.n_large, p.d_cstart, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN);
 }
 
 // k_small and k_medium (independent of the long tasks).
@@ -498,7 +570,7 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
     }
     if ((rc = mark())) return rc;
     hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
-                       p.d_ctl);
+                       p.d_ctl, p.d_work);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
     // k_finish's LDS: cap entries for a task's winners + candidates (a power
@@ -638,11 +710,14 @@ int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel
   // Control blocks without a prediction: pass A only counts.
   // staged without prediction: k_predict only initialises the control blocks
   hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
-                     p.d_ctl);
+                     p.d_ctl, p.d_work);
   RADIK_HIP(hipGetLastError());
-  hipLaunchKernelGGL((k_hist<2, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                     (int)p.n_large, p.total, p.chunk_h, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, 0u);
-  RADIK_HIP(hipGetLastError());
+  if (p.grid_h > 0) {
+    hipLaunchKernelGGL((k_hist<2, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
+                       (int)p.n_large, p.d_cstart, (int)p.n_chunks, p.d_work, p.d_hist, p.d_ctl, p.d_cand,
+                       p.d_bmap + p.off_h, 0u);
+    RADIK_HIP(hipGetLastError());
+  }
   RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),
                            hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl, 0u);
