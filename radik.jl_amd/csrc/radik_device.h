@@ -31,7 +31,7 @@ struct TaskDesc {
                         // also the task's region of the candidate buffer
   int32_t k;            // requested k (>= 0)
   uint32_t flags;       // F_* bits
-  int32_t nblk;         // pass-A blocks whose chunk overlaps the task
+  int32_t nblk;         // pass-A chunks overlapping the task
   int32_t spare;
 };
 static_assert(sizeof(TaskDesc) == 64, "TaskDesc layout");

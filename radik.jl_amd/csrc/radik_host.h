@@ -84,7 +84,6 @@ struct Pipeline {
   int64_t* d_mbases = nullptr;
   uint32_t* d_bmap = nullptr;
   int64_t* d_cstart = nullptr;
-  uint32_t* d_work = nullptr;   // pass A's chunk counter (zeroed by k_predict)
   uint2* d_xmap = nullptr;
   TaskDesc* d_small = nullptr;
   TaskDesc* d_med = nullptr;
@@ -96,8 +95,9 @@ struct Pipeline {
          cap_cstart = 0;
   int64_t n_split = 0;  // k_split blocks (tasks longer than BIG_LEN)
   // Streaming launch shapes fixed at staging (passes A and B share one chunk
-  // table; sampling has equal chunks): pass A's persistent grid, the number
-  // of chunks, the sampling grid, offsets of their maps in the block map.
+  // table, one chunk per block; sampling has equal chunks): the number of
+  // chunks (= pass A's grid), the sampling grid, offsets of their maps in the
+  // block map.
   int64_t n_chunks = 0;
   int grid_h = 0, grid_s = 0;
   int64_t off_h = 0, off_s = 0, n_bmap = 0;

@@ -30,9 +30,8 @@
 // Streaming kernels (k_sample, k_hist, k_compact) split the launch's
 // concatenated element space into chunks, so uneven task lengths balance at
 // element granularity; a chunk that crosses task boundaries flushes per task.
-// k_sample takes equal chunks, one per block; k_hist takes the chunks of a
-// host table, shrinking towards its end, in the order of a device counter;
-// k_compact takes the same table one chunk per block.
+// k_sample takes equal chunks; k_hist and k_compact take the chunks of a host
+// table, shrinking towards its end; one chunk per block.
 
 #include "radik_device.h"
 
@@ -254,11 +253,10 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
 __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ sbases,
                                                    uint32_t* __restrict__ shist,
-                                                   TaskCtl* __restrict__ ctl, uint32_t* __restrict__ work) {
+                                                   TaskCtl* __restrict__ ctl) {
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
   const int t = blockIdx.x;
-  if (t == 0 && threadIdx.x == 0) *work = 0u;  // pass A's chunk counter
   const TaskDesc d = tasks[t];
   const int64_t G = sbases[t + 1] - sbases[t];
   const int64_t m = d.k < d.len ? (int64_t)d.k : d.len;
@@ -524,9 +522,8 @@ __device__ __forceinline__ uint32_t load_dev(const uint32_t* p) {
 // segment, without a global histogram; the others (and every task without
 // SELECT) add their counts to the task's global histogram, selected after the
 // pass.  A task that runs on past the chunk keeps its counts in lh (pend =
-// the task, else -1) until the block's next chunk starts elsewhere, so one
-// long task is flushed about once per block, not once per chunk.  Ends
-// without a barrier.
+// the task, else -1) for the caller to flush, or to extend with a following
+// chunk of the same task.  Ends without a barrier.
 template <int UNR, bool SELECT>
 __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, const int64_t* __restrict__ bases,
                                            int ntasks, int64_t g0, int64_t g1, int t, uint32_t* __restrict__ hist,
@@ -582,16 +579,15 @@ __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, c
   }
 }
 
-// Pass A, persistent: each block takes chunks of the host's chunk table
-// (cstart: nch + 1 starts; cmap: task holding each chunk's start) in the
-// order of a device counter (`work`, zeroed by k_predict), so a block that
-// starts late (its CU busy with other kernels) takes fewer chunks; the
-// table's chunks shrink towards its end, so the blocks finish together.
+// Pass A, one chunk of the host's chunk table per block (cstart: nch + 1
+// starts; cmap: the task holding each chunk's start).  The table's chunks
+// shrink towards its end and blocks start in about block order, so the
+// blocks that start last (as earlier ones finish) carry the least work and
+// the pass ends with the blocks finishing together.
 template <int UNR, bool SELECT>
 __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tasks,
                                                 const int64_t* __restrict__ bases, int ntasks,
-                                                const int64_t* __restrict__ cstart, int nch,
-                                                uint32_t* __restrict__ work,
+                                                const int64_t* __restrict__ cstart,
                                                 uint32_t* __restrict__ hist,
                                                 TaskCtl* __restrict__ ctl,
                                                 uint64_t* __restrict__ cand,
@@ -601,24 +597,14 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
   __shared__ uint32_t stage_n[BLOCK / 64];
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
-  __shared__ int next_chunk;
   for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
   if (threadIdx.x < BLOCK / 64) stage_n[threadIdx.x] = 0;
-  if (threadIdx.x == 0) next_chunk = (int)atomicAdd(work, 1u);
   const WaveStage st{stage_buf[threadIdx.x >> 6], &stage_n[threadIdx.x >> 6]};
+  const int ch = blockIdx.x;
   __syncthreads();
-  int ch = next_chunk;
   int pend = -1;
-  while (ch < nch) {
-    int ch_next = 0;  // fetched while this chunk streams
-    if (threadIdx.x == 0) ch_next = (int)atomicAdd(work, 1u);
-    hist_chunk<UNR, SELECT>(tasks, bases, ntasks, cstart[ch], cstart[ch + 1], (int)cmap[ch], hist, ctl, cand,
-                            fin_cap, lh, st, wsum, res, pend);
-    __syncthreads();  // lh and the stages are free; every thread has read next_chunk
-    if (threadIdx.x == 0) next_chunk = ch_next;
-    __syncthreads();
-    ch = next_chunk;
-  }
+  hist_chunk<UNR, SELECT>(tasks, bases, ntasks, cstart[ch], cstart[ch + 1], (int)cmap[ch], hist, ctl, cand, fin_cap,
+                          lh, st, wsum, res, pend);
   if (pend >= 0) flush_hist(lh, hist + (size_t)pend * HIST_BINS);
 }
 
@@ -1842,10 +1828,10 @@ __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ s
 
 // Explicit instantiations used by the host.
 #define RADIK_INST(U)                                                                                      \
-  template __global__ void k_hist<U, true>(const TaskDesc*, const int64_t*, int, const int64_t*, int, uint32_t*, \
-                                           uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t);           \
-  template __global__ void k_hist<U, false>(const TaskDesc*, const int64_t*, int, const int64_t*, int, uint32_t*, \
-                                            uint32_t*, TaskCtl*, uint64_t*, const uint32_t*, uint32_t);          \
+  template __global__ void k_hist<U, true>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,     \
+                                           TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                      \
+  template __global__ void k_hist<U, false>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,    \
+                                            TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                     \
   template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, const int64_t*, TaskCtl*,         \
                                         uint64_t*, const uint32_t*, int64_t);
 RADIK_INST(1)

@@ -111,7 +111,6 @@ Pipeline::~Pipeline() {
   if (d_bmap) (void)hipFree(d_bmap);
   if (h_cstart) (void)hipHostFree(h_cstart);
   if (d_cstart) (void)hipFree(d_cstart);
-  if (d_work) (void)hipFree(d_work);
   if (h_xmap) (void)hipHostFree(h_xmap);
   if (d_xmap) (void)hipFree(d_xmap);
   if (d_shist) (void)hipFree(d_shist);
@@ -249,21 +248,27 @@ static int64_t sample_groups(int64_t len, int64_t k) {
   return g;
 }
 
-// Pass A's chunk table over `total` elements: guided sizes, each about the
-// remaining elements / (spread * blocks), at least min_chunk and a multiple
-// of gran (a last piece under half of min_chunk joins its predecessor), so
-// the chunks taken last are small and the blocks finish together.  Writes
-// the nch + 1 starts when `out` is non-null; returns nch.
-static int64_t plan_chunks(int64_t total, int64_t blocks, int64_t spread, int64_t gran, int64_t min_chunk,
-                           int64_t* out) {
+// Pass A's chunk table over `total` elements (chunks a multiple of gran, at
+// least min_chunk): equal chunks of about total / blocks, or (guided) chunks
+// of about the remaining elements / (spread * blocks), so the chunks taken
+// last are small and the blocks finish together (a last piece under half of
+// min_chunk joins its predecessor).  Writes the nch + 1 starts when `out` is
+// non-null; returns nch.
+static int64_t plan_chunks(int64_t total, int64_t blocks, int64_t spread, bool guided, int64_t gran,
+                           int64_t min_chunk, int64_t* out) {
   int64_t n = 0, g = 0;
+  const int64_t eq = (std::max((total + blocks - 1) / blocks, min_chunk) + gran - 1) / gran * gran;
   while (g < total) {
     if (out) out[n] = g;
     const int64_t rem = total - g;
-    int64_t c = rem / (spread * blocks);
-    if (c < min_chunk) c = min_chunk;
-    c = (c + gran - 1) / gran * gran;
-    g = (rem - c < min_chunk / 2) ? total : g + c;
+    if (!guided) {
+      g += std::min(eq, rem);
+    } else {
+      int64_t c = rem / (spread * blocks);
+      if (c < min_chunk) c = min_chunk;
+      c = (c + gran - 1) / gran * gran;
+      g = (rem - c < min_chunk / 2) ? total : g + c;
+    }
     ++n;
   }
   if (out) out[n] = total;
@@ -376,16 +381,19 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.max_m_med = (uint32_t)max_m_med;
   // Launch shapes of the streaming kernels, pass A's chunk table and the
   // chunk -> task maps; each long task learns how many chunks share it.
-  // Pass A runs min(blocks_per_cu, resident blocks per CU) persistent blocks
-  // per CU; its table's chunks shrink from remaining / (spread * blocks) down
-  // to min_chunk, spread = how far blocks_per_cu exceeds the resident count.
+  // Pass A's table: with blocks_per_cu up to the resident pass-A blocks per
+  // CU, equal chunks for blocks_per_cu blocks per CU (one round when they
+  // all fit); above it, guided chunks over the resident blocks with spread
+  // blocks_per_cu / (2 * resident), for launches whose blocks start unevenly
+  // (CUs shared with the short and medium tasks' kernels).
   p.grid_h = p.grid_s = 0;
   const int res_cu = std::max(1, f.hist_blocks_per_cu);
-  const int64_t slots = (int64_t)f.cus * std::max(1, std::min(o.blocks_per_cu, res_cu));
-  const int64_t spread = std::max(1, o.blocks_per_cu / res_cu);
+  const bool guided = o.blocks_per_cu > res_cu;
+  const int64_t slots = (int64_t)f.cus * (guided ? res_cu : std::max(1, o.blocks_per_cu));
+  const int64_t spread = guided ? std::max(1, o.blocks_per_cu / (2 * res_cu)) : 1;
   const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
-  const int64_t min_chunk = 8 * gran;
-  const int64_t nch = nl ? plan_chunks(tot, slots, spread, gran, min_chunk, nullptr) : 0;
+  const int64_t min_chunk = (guided ? 8 : 4) * gran;
+  const int64_t nch = nl ? plan_chunks(tot, slots, spread, guided, gran, min_chunk, nullptr) : 0;
   if (nch > 0 && (!p.h_cstart || (size_t)nch + 1 > p.cap_cstart)) {
     RADIK_HIP(hipStreamSynchronize(s));
     size_t c = 64;
@@ -401,13 +409,9 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     }
     p.cap_cstart = c;
   }
-  if (nl > 0 && !p.d_work && hipMalloc((void**)&p.d_work, 256) != hipSuccess) {
-    p.d_work = nullptr;
-    return set_error(6, "allocation of the chunk counter failed");
-  }
-  if (nch > 0) plan_chunks(tot, slots, spread, gran, min_chunk, p.h_cstart);
+  if (nch > 0) plan_chunks(tot, slots, spread, guided, gran, min_chunk, p.h_cstart);
   p.n_chunks = nch;
-  p.grid_h = (int)std::min<int64_t>(nch, slots);
+  p.grid_h = (int)nch;
   p.grid_s = groups > 0 ? (int)((groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK) : 0;
   {
     int64_t c = 0;
@@ -510,8 +514,7 @@ template <int UNR>
 static void launch_stream_kernels_hist(const Pipeline& p, uint32_t fin_cap, hipStream_t s) {
   if (p.grid_h <= 0) return;
   hipLaunchKernelGGL((k_hist<UNR, true>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                     (int)p.n_large, p.d_cstart, (int)p.n_chunks, p.d_work, p.d_hist, p.d_ctl, p.d_cand,
-                     p.d_bmap + p.off_h, fin_cap);
+                     (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
 }
 
 // Pass B over the whole element space (pass A's chunks, one per block);
@@ -570,7 +573,7 @@ int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, 
     }
     if ((rc = mark())) return rc;
     hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
-                       p.d_ctl, p.d_work);
+                       p.d_ctl);
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
     // k_finish's LDS: cap entries for a task's winners + candidates (a power
@@ -710,12 +713,11 @@ int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel
   // Control blocks without a prediction: pass A only counts.
   // staged without prediction: k_predict only initialises the control blocks
   hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
-                     p.d_ctl, p.d_work);
+                     p.d_ctl);
   RADIK_HIP(hipGetLastError());
   if (p.grid_h > 0) {
     hipLaunchKernelGGL((k_hist<2, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                       (int)p.n_large, p.d_cstart, (int)p.n_chunks, p.d_work, p.d_hist, p.d_ctl, p.d_cand,
-                       p.d_bmap + p.off_h, 0u);
+                       (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, 0u);
     RADIK_HIP(hipGetLastError());
   }
   RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),
