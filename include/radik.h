@@ -89,7 +89,9 @@ int radik_plan_execute(radik_plan *plan, float *val_out, void *idx_out, const fl
 int radik_plan_set_graph(radik_plan *plan, int use_graph);
 /* Tuning knobs (results do not depend on them): vload_bytes 16/32/64 (bytes
  * in flight per lane per step of the streaming passes, 0 = keep), streaming
- * blocks per CU (0 = keep), longest task run whole in one block, candidates
+ * blocks per CU (0 = keep; up to the pass's resident blocks per CU: equal
+ * chunks, one per block; above it: chunks shrinking towards the end of the
+ * pass), longest task run whole in one block, candidates
  * held in LDS by the finishing kernel of launches with many tasks (0 = device
  * defaults), and whether long tasks are sampled so one read of the input
  * suffices. */

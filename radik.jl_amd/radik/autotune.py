@@ -1,9 +1,10 @@
 """Autotuner for the batched plan's launch options.
 
 Picks, per length bucket, the vector-load width (bytes in flight per lane per
-step of the streaming passes), the streaming blocks per CU, the longest task
-run whole in one block (k_small) and the candidates held in LDS by the
-finishing kernel.  Results do not depend on these options, only speed does.
+step of the streaming passes), the streaming blocks per CU (up to the
+resident count equal chunks, above it chunks shrinking towards the end of
+the pass), the longest task run whole in one block (k_small) and the
+candidates held in LDS by the finishing kernel.  Results do not depend on these options, only speed does.
 
 The search is coordinate descent: one knob at a time over its candidates,
 keeping the best, timed with HIP events over a few executions of the plan on
