@@ -29,7 +29,7 @@ struct DevFacts {
   int cus = 256;
   size_t lds_per_block = 65536;
   uint32_t small_cap = 2048;   // default: longest task routed to k_small
-  uint32_t refine_cap = 4096;  // default: candidates kept in LDS by k_refine
+  uint32_t refine_cap = 4096;  // default: candidates kept in LDS by k_finish
   uint32_t small_cap_max = 2048, refine_cap_max = 4096;  // what the LDS budget allows
   uint32_t sort_cap = 4096;    // results sorted in LDS by k_sort
   int hist_blocks_per_cu = 4;  // resident pass-A blocks per CU (occupancy of k_hist)
@@ -51,7 +51,7 @@ struct LaunchOpts {
   int unr = 2;              // float4 loads in flight per thread (vload bytes / 16)
   int blocks_per_cu = 8;    // streaming grid: blocks per CU
   uint32_t small_cap = 0;   // longest task for k_small (0: device default)
-  uint32_t refine_cap = 0;  // candidates held in LDS by k_refine (0: device default)
+  uint32_t refine_cap = 0;  // candidates held in LDS by k_finish (0: device default)
   bool predict = true;      // sample long tasks and spill in pass A
   bool medium = true;       // medium tasks to k_medium (else the streaming passes)
 };
@@ -133,7 +133,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
                 hipStream_t s);
 
 // Phase timing hooks (events recorded between kernels when non-null).
-// Phases: sample, predict, hist, select, compact, refine, sort, small.
+// Phases: sample, predict, hist, select, compact, finish, sort, small.
 constexpr int NUM_PHASES = 8;
 struct PhaseTimer {
   hipEvent_t ev[NUM_PHASES + 1] = {};

@@ -4,28 +4,37 @@
 // smaller is better, so the top-k of a task are its k smallest composites and
 // ties at the k-th value fall to the lowest position.  The first digit
 // (rank-key bits [31:20], the reference's pass-1 digit, count_bin.jl:78-203,
-// algorithm.jl:221-224) is resolved over the raw input by two streaming passes:
+// algorithm.jl:221-224) is resolved over the raw input, which is read once in
+// the common case:
 //
 //   k_sample   first-digit histogram of an evenly spread ~1/64 sample of each
 //              long task
 //   k_predict  per task: a first-digit bound b_pred that the task's m best
 //              keys stay under with high probability
 //   k_hist     pass A, the one full read: exact 4096-bin histogram per task
-//              (LDS ds_add, one global atomic per non-zero bin per block) and,
+//              (LDS ds_add, one global atomic per non-zero bin per chunk) and,
 //              in the same read, a spill of every key with digit <= b_pred
-//              (wave ballot + one atomic per wave)            ~ count_bin_ex_kernel!
-//   k_select   block scan of the 4096 bins -> winner bound, candidate bin,
-//              residual k; checks the spill covers the bin  ~ select_bin_kernel!
-//   k_compact  pass B, only for tasks whose prediction was missing or too
-//              tight: re-reads them, winners (digit < bin) to the output slots,
-//              candidates (digit == bin) to the spill    ~ select_candidate_ex_kernel!
-//   k_refine   per task: emits the spilled winners, then resolves the
-//              remaining digits (rank bits [19:8], [7:0] as in
+//              (per-wave LDS staging); tasks inside one chunk are selected at
+//              once from the LDS histogram               ~ count_bin_ex_kernel!
+//   k_select   (tasks over 2^20 elements) block scan of the 4096 bins ->
+//              winner bound, candidate bin, residual k   ~ select_bin_kernel!
+//   k_compact  pass B, only for tasks over 2^20 elements whose prediction was
+//              missing or too tight: re-reads them, winners (digit < bin) to
+//              the output slots, candidates (digit == bin) to the spill
+//                                                       ~ select_candidate_ex_kernel!
+//   k_split    (tasks over 2^20 elements) lists winners and candidates out of
+//              the spill with several blocks per task
+//   k_finish   per long task: the selection when pass A did not make it, then
+//              the remaining digits (rank bits [19:0] as in
 //              algorithm.jl:247-299, then position digits for exact ties) over
-//              the candidates, in LDS when they fit          ~ passes 2/3 + filter
-//   k_sort     per task: bitonic sort of the k results by (output key,
-//              position), indices gathered                    ~ bitonic_sort!/merge_sort
-//   k_small    one block per short task: everything above in LDS, one launch.
+//              the candidates (in LDS, or bounded by a sample when there are
+//              many), the winners and chosen candidates emitted and sorted in
+//              registers                           ~ passes 2/3 + filter + sort
+//   k_sort     per task, when the results exceed k_finish's LDS: bitonic sort
+//              of the k results by (output key, position) ~ bitonic_sort!/merge_sort
+//   k_small    one block per short task: everything above in LDS.
+//   k_medium   one block per medium task: LDS histogram of one read, the
+//              selection, then k_finish's code on a re-read.
 //
 // Streaming kernels (k_sample, k_hist, k_compact) split the launch's
 // concatenated element space into chunks, so uneven task lengths balance at
@@ -753,7 +762,7 @@ __device__ __forceinline__ void for_each_entry(Src src, uint32_t n, F&& f) {
 }
 
 // ---------------------------------------------------------------------------
-// Digit selection over composites (shared by k_refine and k_small).
+// Digit selection over composites (shared by the finishing code and k_small).
 // Digits below the first: rank bits [19:8], [7:0], then position bits
 // [31:20], [19:8], [7:0].  Starting from prefix P (the composite bits above
 // `top` already fixed) and need = how many of the prefix-matching composites
@@ -824,9 +833,9 @@ __device__ __forceinline__ void block_and_or(uint64_t a, uint64_t o, unsigned lo
 }
 
 // ---------------------------------------------------------------------------
-// Refinement, one block per task: emits the winners held in the spill (pass A
-// spills), then resolves the candidates of the threshold bin digit by digit
-// (in LDS when they fit) and emits the chosen ones.
+// Helpers of the finishing code (k_finish, k_medium): emitting winners held
+// in the spill, resolving the candidates of the threshold bin digit by digit
+// (in LDS when they fit) and emitting the chosen ones.
 // ---------------------------------------------------------------------------
 
 // Value of a spilled composite: the original element.
