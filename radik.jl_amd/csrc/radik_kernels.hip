@@ -1395,11 +1395,14 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   // two composites L <= U read off a sample (as pass A bounds the first
   // digit), then one pass emits the winners and the candidates below L and
   // keeps those in [L, U] in LDS; checked, else the exact digits below.
-  const uint64_t* __restrict__ ssrc = spilled ? region : lst;
-  const uint32_t nss = spilled ? n_sp : nlst;
-  const bool sampled = !fin && (spilled || listed) && krem > 0 && ncl > cap;
+  // A re-read here whose candidates exceed LDS leaves them at the start of
+  // the spill region: they are bounded the same way after the re-read.
+  const bool reread_sampled = !fin && !spilled && !listed && !cand_lds && krem > 0;
+  const bool sampled = (!fin && (spilled || listed) && krem > 0 && ncl > cap) || reread_sampled;
+  const uint64_t* ssrc = reread_sampled ? region : (spilled ? region : lst);
+  const uint32_t nss = reread_sampled ? ncand : (spilled ? n_sp : nlst);
   bool exact = true;
-  if (sampled) {
+  auto bound_by_sample = [&]() {
     // every stride-th entry, at most `want` of them (all kept: a sample cut
     // short would cover only the source's first part, whose positions need
     // not be typical)
@@ -1516,6 +1519,9 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
       o = ~0ull;
       if (threadIdx.x == 0) ectr = 0;
     }
+  };
+  if (sampled && !reread_sampled) {
+    // bounded below, straight from the spill or the list
   } else if (listed) {
     // fin: winners and candidates into LDS; else candidates (into LDS when
     // they fit, else they stay listed)
@@ -1555,6 +1561,20 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         take(((uint64_t)rk << 32) | pos, hit && dg < bw, hit && dg == bc);
       }
     });
+  }
+  if (sampled) {
+    if (reread_sampled) {
+      __syncthreads();  // the candidates are in the spill region
+      if (threadIdx.x == 0) {
+        cctr = 0;
+        vb[0] = ~0ull;
+        vb[1] = 0ull;
+      }
+      __syncthreads();
+      a = ~0ull;
+      o = 0ull;
+    }
+    bound_by_sample();
   }
   __syncthreads();
   if (exact) block_and_or(a, o, vb);

@@ -320,6 +320,26 @@ def test_heavy_ties_at_the_threshold():
     assert_same(run_ws(xs, lens, 256, O.LARGEST), oracle_topk(xs, lens, 256, O.LARGEST), "tie batch")
 
 
+def test_heavy_ties_reread_tasks():
+    # Tie-heavy tasks that are read a second time inside one block (medium
+    # tasks beside the passes, and long unsampled ones: k > len / 64): more
+    # candidates than the block's LDS, bounded by a sample of the candidates
+    # the re-read left in the spill region.
+    rng = np.random.default_rng(4343)
+    lens = [40000, 50000, 65000, 3000, 30000]
+    xs = rng.standard_normal(sum(lens)).astype(np.float32)
+    xs[rng.random(xs.size) < 0.08] = np.float32(3.0)
+    for k, flags in ((256, O.LARGEST), (100, O.LARGEST | O.ASCEND), (256, O.LARGEST | O.WITHSCALE), (1000, 0)):
+        say("medium ties", k, flags)
+        assert_same(run_ws(xs, lens, k, flags), oracle_topk(xs, lens, k, flags), f"medium ties k{k} f{flags}")
+    n = 200000
+    x = rng.standard_normal(n).astype(np.float32)
+    x[rng.random(n) < 0.10] = np.float32(2.2)  # ~20000 ties, beyond a single task's 16384 LDS entries
+    for k in (4000, 9000):
+        say("unsampled ties", k)
+        assert_same(run_ws(x, [n], k, O.LARGEST), oracle_topk(x, [n], k, O.LARGEST), f"unsampled ties k{k}")
+
+
 def test_large_k_multi_block_sort():
     # k above 4096 sorts in several blocks (runs + ranks); ties and specials
     rng = np.random.default_rng(16384)
