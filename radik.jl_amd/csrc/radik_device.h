@@ -18,6 +18,9 @@ constexpr uint32_t F_IDX64 = 0x100u;
 constexpr uint32_t NO_BIN = 0xFFFFFFFFu;
 constexpr int HIST_BINS = 4096;  // first digit: rank-key bits [31:20]
 constexpr int BLOCK = 256;       // threads of every kernel in this library
+// Predictions are refined to a composite bound only for samples of at most
+// this many groups (the host picks pass A's variant by it).
+constexpr int PRED_MAX_G = 64;
 
 // One task of a launch.  Pointers address the task's own first element, so a
 // launch may mix tasks from different buffers (the serving engine does).
@@ -47,12 +50,14 @@ struct TaskCtl {
   uint32_t m;         // min(k, len): valid outputs
   uint32_t win_ctr;   // slot counter for winners (compaction)
   uint32_t cand_ctr;  // entries in the task's spill region
-  uint32_t b_pred;    // predicted bound: pass A spills first digits <= b_pred (NO_BIN: no spill)
-  uint32_t spill_all; // 1: the spill holds winners and candidates; 0: pass B re-reads the task
-  uint32_t fin;       // 1: winners + candidates fit k_finish's LDS (nwin + ncand <= its capacity)
+  uint32_t b_pred;    // predicted first digit (NO_BIN: no prediction, no spill)
+  uint32_t spill_all; // 1: the spill holds the m best; 0: the task is re-read
+  uint32_t fin;       // 1: the m best are gathered in k_finish's LDS (their source fits it)
   uint32_t done;      // 1: the first digit is selected
   uint32_t list_ctr;  // candidates listed by k_split
-  uint32_t pad[3];
+  uint32_t r_pred;    // pass A spills the composites <= (r_pred << 32 | p_pred)
+  uint32_t p_pred;
+  uint32_t pad;
 };
 static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 

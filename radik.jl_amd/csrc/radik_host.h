@@ -106,6 +106,7 @@ struct Pipeline {
   int64_t n_med = 0;
   uint32_t max_m_med = 0;
   bool has_big = false;    // some long task exceeds BIG_LEN (pass B over several blocks)
+  bool pos_bounds = false; // some predicted task may get a spill bound with a position (PRED_MAX_G)
   uint32_t max_m = 0;  // largest min(k, len) over the long tasks
   int64_t total_runs = 0, total_m = 0;  // multi-block sort sizes
   hipEvent_t upload_done = nullptr;  // pinned buffers reusable after this

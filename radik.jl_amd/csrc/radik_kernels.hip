@@ -200,6 +200,81 @@ __device__ __forceinline__ void flush_hist(uint32_t* __restrict__ lh, uint32_t* 
   }
 }
 
+// Block-wide AND and OR of the composites each thread saw (a, o): vb[0] and
+// vb[1] in LDS, initialised by the caller to ~0 and 0 before a barrier.
+__device__ __forceinline__ void block_and_or(uint64_t a, uint64_t o, unsigned long long* vb) {
+  atomicAnd(&vb[0], (unsigned long long)a);
+  atomicOr(&vb[1], (unsigned long long)o);
+  __syncthreads();
+}
+
+// Bounds L <= U with L at most the ql-th and U at least the qh-th smallest
+// (1 <= ql <= qh <= n) of n distinct composites v[0, n) in LDS that share the
+// first digit `top`, found together over the narrow digits: one histogram
+// per digit while both lie under one prefix, one each after they part.  The
+// lower side stops at a bin whose smallest member is its quantile (L = the
+// lowest composite of that bin), the upper one at a bin whose largest member
+// is (U = the highest composite of that bin), so no sample entry lies in
+// [L, U] beyond the ql-th .. qh-th smallest.
+__device__ __forceinline__ void sample_bounds(const uint64_t* v, uint32_t n, uint32_t ql, uint32_t qh, uint32_t top,
+                                              uint64_t vary, uint64_t same, uint32_t* lh8, uint32_t* wsum,
+                                              uint32_t* res, uint64_t& L, uint64_t& U) {
+  constexpr int shifts[7] = {44, 36, 32, 24, 16, 8, 0};
+  constexpr int widths[7] = {8, 8, 4, 8, 8, 8, 8};
+  uint64_t Pl = top, Ph = top;
+  int sl = 52, sh = 52;
+  bool dl = false, dh = false;
+  for (int q = 0; q < 7 && !(dl && dh); ++q) {
+    const int s = shifts[q], w = widths[q];
+    const uint32_t nb = 1u << w;
+    const int hs = s + w;
+    if (((vary >> s) & (nb - 1)) == 0) {  // one bin
+      const uint64_t fix = (same >> s) & (nb - 1);
+      if (!dl) {
+        Pl = (Pl << w) | fix;
+        sl = s;
+      }
+      if (!dh) {
+        Ph = (Ph << w) | fix;
+        sh = s;
+      }
+      continue;
+    }
+    const bool shared = !dl && !dh && Pl == Ph;
+    uint32_t c[1] = {0u};
+    for (int side = 0; side < 2; ++side) {
+      if (side == 0 ? dl : dh) continue;
+      const uint64_t P = side == 0 ? Pl : Ph;
+      if (!(shared && side == 1)) {  // (the shared histogram serves both)
+        for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) lh8[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+          const uint64_t x = v[i];
+          if ((x >> hs) == P) hist_add(lh8, (uint32_t)(x >> s) & (nb - 1));
+        }
+        __syncthreads();
+        c[0] = (threadIdx.x < nb) ? lh8[threadIdx.x] : 0u;
+      }
+      block_find_bin<1>(c, side == 0 ? ql : qh, wsum, res);
+      const uint32_t b = res[0], before = res[1], cnt = res[2];
+      __syncthreads();  // res and lh8 are rewritten next
+      if (side == 0) {
+        Pl = (Pl << w) | b;
+        sl = s;
+        ql -= before;
+        dl = ql == 1;
+      } else {
+        Ph = (Ph << w) | b;
+        sh = s;
+        qh -= before;
+        dh = qh == cnt;
+      }
+    }
+  }
+  L = Pl << sl;
+  U = ((Ph + 1ull) << sh) - 1ull;  // (wraps to ~0 at the top)
+}
+
 // ---------------------------------------------------------------------------
 // Sampling: a predicted tasks owns G groups of SAMPLE_GROUP consecutive
 // elements spread evenly over it (group j starts at j * len / G).  Blocks take
@@ -253,23 +328,45 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
 
 // ---------------------------------------------------------------------------
 // Prediction, one block per task; initialises the task's control block.
-// From the sample histogram it picks the smallest first digit b_pred such
-// that the sample holds at least q = f*m + 4*sqrt(f*m) + 4 keys with digit
-// <= b_pred (f = sampled fraction, m = min(k, len)), so the task's m best
-// keys lie at or below b_pred unless the sample is off by four standard
-// deviations; a miss only costs the fallback re-read.  Clears the sample bins.
+// From the sample histogram it picks the first digit b_pred of the q-th
+// smallest sampled key, q = f*m + 4*sqrt(f*m) + 4 (f = sampled fraction,
+// m = min(k, len)); then, re-reading the sampled elements of that digit
+// (when the digit holds more than PRED_MIN and at most PRED_CAP of them, in
+// a sample of at most PRED_MAX_G groups),
+// it bounds the q-th smallest sampled composite (rank << 32 | position) from
+// above by (r_pred, p_pred), the position rounded up to the end of the gap
+// after its sample group (the sample is contiguous groups, so positions are
+// only known per group).  Pass A spills exactly the composites at or below
+// that bound; the task holds at least m of them unless the sample is off by
+// about four standard deviations, and then they include its m best (a miss
+// only costs the fallback re-read).  Within a digit the bound follows
+// positions too, so an exact tie across the threshold spills only about as
+// many entries as needed.  Clears the sample bins.
 // ---------------------------------------------------------------------------
+constexpr uint32_t PRED_MIN = 32;
+constexpr uint32_t PRED_CAP = 1024;
+
 __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ sbases,
                                                    uint32_t* __restrict__ shist,
                                                    TaskCtl* __restrict__ ctl) {
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
+  __shared__ uint32_t lh8[256];
+  __shared__ uint32_t nsel;
+  __shared__ unsigned long long vb[2];
+  __shared__ uint64_t sv[PRED_CAP];
   const int t = blockIdx.x;
   const TaskDesc d = tasks[t];
   const int64_t G = sbases[t + 1] - sbases[t];
   const int64_t m = d.k < d.len ? (int64_t)d.k : d.len;
-  if (threadIdx.x == 0) res[0] = NO_BIN;
+  if (threadIdx.x == 0) {
+    res[0] = NO_BIN;
+    nsel = 0;
+    vb[0] = ~0ull;
+    vb[1] = 0ull;
+  }
+  uint32_t need = 0;
   if (G > 0) {
     uint4* __restrict__ h4 = reinterpret_cast<uint4*>(shist + (size_t)t * HIST_BINS) + threadIdx.x * 4;
     uint32_t c[16];
@@ -285,15 +382,80 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
     const float S = (float)(G * SAMPLE_GROUP);
     const float mu = S / (float)d.len * (float)m;
     const float q = mu + 4.0f * sqrtf(mu) + 4.0f;
-    const uint32_t need = (uint32_t)ceilf(q);
+    need = (uint32_t)ceilf(q);
     if (m > 0 && (float)need <= S) block_find_bin<16>(c, need, wsum, res);
   }
   __syncthreads();
+  const uint32_t b = res[0], before = res[1], cnt = res[2];
+  uint32_t rp = 0, pp = 0;
+  if (b < (uint32_t)HIST_BINS) {
+    rp = digit_bound(b);  // the whole digit, unless refined below
+    pp = 0xFFFFFFFFu;
+    if (cnt > PRED_MIN && cnt <= PRED_CAP && G <= PRED_MAX_G) {
+      // the sampled elements of digit b as composites (the groups of k_sample,
+      // 32 groups in flight at a time), placed by block scans
+      const float sc = task_scaler(d);
+      const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
+      const RADIK_AS1 float* __restrict__ in = glb(d.in);
+      uint64_t a = ~0ull, o = 0ull;
+      uint32_t base = 0;  // entries placed by the earlier halves
+      for (int h = 0; h < PRED_MAX_G; h += 32) {
+        if (h >= G) break;
+        float xv[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+          const int64_t i = (int64_t)(h + u) * d.len / G + threadIdx.x;
+          xv[u] = (h + u < G && i < d.len) ? in[i] : 0.0f;
+        }
+        uint32_t hm = 0;
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+          const int64_t i = (int64_t)(h + u) * d.len / G + threadIdx.x;
+          if (h + u < G && i < d.len && (rank_key(xv[u], sc, xm) >> 20) == b) hm |= 1u << u;
+        }
+        const uint32_t mine = (uint32_t)__builtin_popcount(hm);
+        const uint32_t incl = block_incl_scan(mine, wsum);
+        if (threadIdx.x == BLOCK - 1) nsel = base + incl;
+        uint32_t slot = base + incl - mine;
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+          if ((hm >> u) & 1u) {
+            const uint32_t pos = (uint32_t)((int64_t)(h + u) * d.len / G + threadIdx.x);
+            const uint64_t x = ((uint64_t)rank_key(xv[u], sc, xm) << 32) | pos;
+            if (slot < PRED_CAP) sv[slot] = x;
+            ++slot;
+            a &= x;
+            o |= x;
+          }
+        }
+        __syncthreads();
+        base = nsel;
+      }
+      __syncthreads();
+      block_and_or(a, o, vb);
+      if (nsel == cnt && need > before && need - before <= cnt) {
+        const uint32_t qi = need - before;  // the q-th smallest is the qi-th of digit b
+        uint64_t L, Ub;
+        sample_bounds(sv, cnt, qi, qi, b, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh8, wsum, res, L, Ub);
+        rp = (uint32_t)(Ub >> 32);
+        pp = (uint32_t)Ub;
+        if (pp != 0xFFFFFFFFu) {  // up to the position before the next group's start
+          int64_t j = (int64_t)pp * G / d.len;
+          if (j > G - 1) j = G - 1;
+          while (j > 0 && j * d.len / G > (int64_t)pp) --j;
+          while (j + 1 < G && (j + 1) * d.len / G <= (int64_t)pp) ++j;
+          pp = (j + 1 < G) ? (uint32_t)((j + 1) * d.len / G - 1) : 0xFFFFFFFFu;
+        }
+      }
+    }
+  }
   if (threadIdx.x == 0) {
     TaskCtl o = {};
     o.b_win = 0;
     o.b_cand = NO_BIN;
-    o.b_pred = (res[0] < (uint32_t)HIST_BINS) ? res[0] : NO_BIN;
+    o.b_pred = (b < (uint32_t)HIST_BINS) ? b : NO_BIN;
+    o.r_pred = rp;
+    o.p_pred = pp;
     ctl[t] = o;
   }
 }
@@ -302,10 +464,10 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
 // First-digit selection of one task from its complete 4096-bin histogram (c:
 // this thread's 16 consecutive bins), called by the whole block.  Decides the
 // winner bound, the candidate bin and the residual k, whether the spill of
-// pass A holds every key the task needs (first digit <= the selected bin) or
-// the task is re-read, and whether its winners and candidates fit k_finish's
-// LDS (fin_cap entries).  b_pred / spilled: the prediction and the spill
-// count of pass A.  Thread 0 writes the control block.
+// pass A holds the task's m best or the task is re-read, and whether their
+// source (the spill, or winners + the candidate bin) fits k_finish's LDS
+// (fin_cap entries).  b_pred / spilled: the prediction and the spill count of
+// pass A.  Thread 0 writes the control block.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void select_first(const TaskDesc& d, const uint32_t (&c)[16], uint32_t b_pred,
                                              uint32_t spilled, uint32_t fin_cap, TaskCtl* out,
@@ -345,21 +507,22 @@ __device__ __forceinline__ void select_first(const TaskDesc& d, const uint32_t (
     o.nwin = o.m - o.krem;
     o.win_ctr = 0;
     o.done = 1;
-    // Highest first digit the task needs: the candidate bin, or the last
-    // winner bin when the threshold bin is taken whole.
-    const uint32_t need = (o.b_cand != NO_BIN) ? o.b_cand : (o.b_win - 1);
-    const uint64_t both = (uint64_t)o.nwin + o.ncand;
-    o.fin = (o.m > 0 && both <= fin_cap) ? 1u : 0u;
+    // The spill is every composite at or below the predicted bound; with at
+    // least m entries it holds the task's m best.  Their source for k_finish
+    // is then the spill (n_sp entries), else a re-read (winners and the whole
+    // candidate bin).
     if (o.m == 0) {
       o.spill_all = 1;
       o.cand_ctr = 0;
-    } else if (b_pred == NO_BIN || need > b_pred || spilled > (uint32_t)len) {
+    } else if (b_pred == NO_BIN || spilled < o.m || spilled > (uint32_t)len) {
       o.spill_all = 0;
       o.cand_ctr = 0;
     } else {
       o.spill_all = 1;
       o.cand_ctr = spilled;
     }
+    const uint64_t src = o.spill_all ? (uint64_t)o.cand_ctr : (uint64_t)o.nwin + o.ncand;
+    o.fin = (o.m > 0 && src <= fin_cap) ? 1u : 0u;
     *out = o;
   }
 }
@@ -417,13 +580,20 @@ __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ t
 
 // ---------------------------------------------------------------------------
 // Pass A: exact first-digit histogram of every task; for predicted tasks the
-// same read also spills every key with first digit <= b_pred (composite
-// rank << 32 | position) into the task's spill region.
+// same read also spills every composite (rank << 32 | position) at or below
+// the predicted bound (r_pred, p_pred) into the task's spill region.
 // ---------------------------------------------------------------------------
-template <int UNR, bool SCALE, bool SPILL>
+template <int UNR, bool SCALE, bool SPILL, bool POS>
 __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int64_t e, float sc, uint32_t xm,
-                                               uint32_t smax, uint32_t* lh, uint32_t* ctr,
+                                               uint32_t rp, uint32_t pp, uint32_t* lh, uint32_t* ctr,
                                                uint64_t* __restrict__ spill, const WaveStage& st) {
+  // spilled: composites (rank << 32 | position) at or below (rp << 32 | pp);
+  // without POS the bound takes every position of rank rp (pp = 0xFFFFFFFF)
+  auto spills = [&](uint32_t rk, uint32_t pos) {
+    if (POS) return rk < rp || (rk == rp && pos <= pp);
+    (void)pos;
+    return rk <= rp;
+  };
   const uint32_t slen = (uint32_t)d.len;
   auto write = [&](uint32_t slot, uint64_t v) {
     if (slot < slen) glb(spill)[slot] = v;
@@ -431,7 +601,7 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
   auto one = [&](float x, uint32_t pos) {
     const uint32_t rk = rk_of<SCALE>(x, sc, xm);
     atomicAdd(&lh[rk >> 20], 1u);
-    if (SPILL) stage_append(st, rk <= smax, ((uint64_t)rk << 32) | pos, ctr, write);
+    if (SPILL) stage_append(st, spills(rk, pos), ((uint64_t)rk << 32) | pos, ctr, write);
   };
   // Scalar head up to 16-byte alignment and tail, as in stream_range; the
   // float4 body spills per iteration: the UNR*4 hits of each lane (a bit
@@ -459,50 +629,64 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
       for (int j = 0; j < 4; ++j) {
         const uint32_t rk = rk_of<SCALE>(r[u][j], sc, xm);
         atomicAdd(&lh[rk >> 20], 1u);
-        if (SPILL && rk <= smax) hit |= 1u << (4 * u + j);
+        if (SPILL && rk <= rp) hit |= 1u << (4 * u + j);
       }
     }
     if (SPILL && ballot(hit != 0)) {
-      uint64_t mk[4 * UNR];
-      uint32_t total = 0;
+      // A rank equal to rp is a hit only up to position pp (checked here, in
+      // iterations with hits only, so the common path keeps its one compare).
+      bool any = true;
+      if (POS) {
+        const uint32_t p0 = (uint32_t)(a + (v << 2));  // position of this lane's first element
 #pragma unroll
-      for (int b = 0; b < 4 * UNR; ++b) {
-        mk[b] = ballot((hit >> b) & 1u);
-        total += (uint32_t)__builtin_popcountll(mk[b]);
+        for (int b = 0; b < 4 * UNR; ++b)
+          if (((hit >> b) & 1u) && rk_of<SCALE>(r[b >> 2][b & 3], sc, xm) == rp &&
+              p0 + (uint32_t)((b >> 2) * BLOCK * 4 + (b & 3)) > pp)
+            hit &= ~(1u << b);
+        any = ballot(hit != 0) != 0;
       }
-      // The stage keeps stage_append's invariant: at most STAGE - 64 entries
-      // between appends.
-      lds_wave_sync();
-      uint32_t n0 = *st.n;
-      if (n0 + total > STAGE - 64) {
-        stage_flush(st, ctr, write);
-        n0 = 0;
-      }
-      uint32_t off;
-      const bool direct = total > STAGE - 64;  // more than a stage holds: straight to the spill
-      if (direct) {
-        uint32_t base = 0;
-        if (lanes_below(ballot(true)) == 0) base = atomicAdd(ctr, total);
-        off = __builtin_amdgcn_readfirstlane(base);
-      } else {
-        off = n0;
-      }
+      if (any) {
+        uint64_t mk[4 * UNR];
+        uint32_t total = 0;
 #pragma unroll
-      for (int b = 0; b < 4 * UNR; ++b) {
-        if ((hit >> b) & 1u) {
-          const int u = b >> 2, j = b & 3;
-          const uint32_t pos = (uint32_t)(a + ((v + (int64_t)u * BLOCK) << 2)) + (uint32_t)j;
-          const uint64_t ent = ((uint64_t)rk_of<SCALE>(r[u][j], sc, xm) << 32) | pos;
-          const uint32_t slot = off + lanes_below(mk[b]);
-          if (direct) write(slot, ent);
-          else st.buf[slot] = ent;
+        for (int b = 0; b < 4 * UNR; ++b) {
+          mk[b] = ballot((hit >> b) & 1u);
+          total += (uint32_t)__builtin_popcountll(mk[b]);
         }
-        off += (uint32_t)__builtin_popcountll(mk[b]);
-      }
-      if (!direct) {
+        // The stage keeps stage_append's invariant: at most STAGE - 64 entries
+        // between appends.
         lds_wave_sync();
-        if (lanes_below(ballot(true)) == 0) *st.n = n0 + total;
-        lds_wave_sync();
+        uint32_t n0 = *st.n;
+        if (n0 + total > STAGE - 64) {
+          stage_flush(st, ctr, write);
+          n0 = 0;
+        }
+        uint32_t off;
+        const bool direct = total > STAGE - 64;  // more than a stage holds: straight to the spill
+        if (direct) {
+          uint32_t base = 0;
+          if (lanes_below(ballot(true)) == 0) base = atomicAdd(ctr, total);
+          off = __builtin_amdgcn_readfirstlane(base);
+        } else {
+          off = n0;
+        }
+#pragma unroll
+        for (int b = 0; b < 4 * UNR; ++b) {
+          if ((hit >> b) & 1u) {
+            const int u = b >> 2, j = b & 3;
+            const uint32_t pos = (uint32_t)(a + ((v + (int64_t)u * BLOCK) << 2)) + (uint32_t)j;
+            const uint64_t ent = ((uint64_t)rk_of<SCALE>(r[u][j], sc, xm) << 32) | pos;
+            const uint32_t slot = off + lanes_below(mk[b]);
+            if (direct) write(slot, ent);
+            else st.buf[slot] = ent;
+          }
+          off += (uint32_t)__builtin_popcountll(mk[b]);
+        }
+        if (!direct) {
+          lds_wave_sync();
+          if (lanes_below(ballot(true)) == 0) *st.n = n0 + total;
+          lds_wave_sync();
+        }
       }
     }
   }
@@ -533,7 +717,7 @@ __device__ __forceinline__ uint32_t load_dev(const uint32_t* p) {
 // pass.  A task that runs on past the chunk keeps its counts in lh (pend =
 // the task, else -1) for the caller to flush, or to extend with a following
 // chunk of the same task.  Ends without a barrier.
-template <int UNR, bool SELECT>
+template <int UNR, bool SELECT, bool POSK>
 __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, const int64_t* __restrict__ bases,
                                            int ntasks, int64_t g0, int64_t g1, int t, uint32_t* __restrict__ hist,
                                            TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
@@ -550,18 +734,20 @@ __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, c
     const int64_t se = min(g1, tb + d.len);
     const float sc = task_scaler(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-    const uint32_t bp = ctl[t].b_pred;
+    const uint32_t bp = ctl[t].b_pred, rp = ctl[t].r_pred, pp = ctl[t].p_pred;
     const bool scale = (d.flags & F_WITHSCALE) != 0;
     uint32_t* ctr = &ctl[t].cand_ctr;
     uint64_t* spill = cand + d.base;
     const int64_t s = g0 - tb, e = se - tb;
     if (bp == NO_BIN) {
-      if (scale) pass_a_segment<UNR, true, false>(d, s, e, sc, xm, 0u, lh, ctr, spill, st);
-      else pass_a_segment<UNR, false, false>(d, s, e, sc, xm, 0u, lh, ctr, spill, st);
+      if (scale) pass_a_segment<UNR, true, false, false>(d, s, e, sc, xm, 0u, 0u, lh, ctr, spill, st);
+      else pass_a_segment<UNR, false, false, false>(d, s, e, sc, xm, 0u, 0u, lh, ctr, spill, st);
+    } else if (!POSK || pp == 0xFFFFFFFFu) {
+      if (scale) pass_a_segment<UNR, true, true, false>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
+      else pass_a_segment<UNR, false, true, false>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
     } else {
-      const uint32_t smax = digit_bound(bp);
-      if (scale) pass_a_segment<UNR, true, true>(d, s, e, sc, xm, smax, lh, ctr, spill, st);
-      else pass_a_segment<UNR, false, true>(d, s, e, sc, xm, smax, lh, ctr, spill, st);
+      if (scale) pass_a_segment<UNR, true, true, POSK>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
+      else pass_a_segment<UNR, false, true, POSK>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
     }
     __syncthreads();
     if (se >= g1 && se < tb + d.len) {
@@ -592,8 +778,10 @@ __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, c
 // starts; cmap: the task holding each chunk's start).  The table's chunks
 // shrink towards its end and blocks start in about block order, so the
 // blocks that start last (as earlier ones finish) carry the least work and
-// the pass ends with the blocks finishing together.
-template <int UNR, bool SELECT>
+// the pass ends with the blocks finishing together.  POSK: some task's
+// spill bound may hold a position (a launch without one runs the kernel
+// without that code, which keeps its registers for the streaming loop).
+template <int UNR, bool SELECT, bool POSK>
 __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tasks,
                                                 const int64_t* __restrict__ bases, int ntasks,
                                                 const int64_t* __restrict__ cstart,
@@ -612,7 +800,8 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
   const int ch = blockIdx.x;
   __syncthreads();
   int pend = -1;
-  hist_chunk<UNR, SELECT>(tasks, bases, ntasks, cstart[ch], cstart[ch + 1], (int)cmap[ch], hist, ctl, cand, fin_cap,
+  hist_chunk<UNR, SELECT, POSK>(tasks, bases, ntasks, cstart[ch], cstart[ch + 1], (int)cmap[ch], hist, ctl, cand,
+                                fin_cap,
                           lh, st, wsum, res, pend);
   if (pend >= 0) flush_hist(lh, hist + (size_t)pend * HIST_BINS);
 }
@@ -824,13 +1013,6 @@ __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint6
   return s_final;
 }
 
-// Block-wide AND and OR of the composites each thread saw (a, o): vb[0] and
-// vb[1] in LDS, initialised by the caller to ~0 and 0 before a barrier.
-__device__ __forceinline__ void block_and_or(uint64_t a, uint64_t o, unsigned long long* vb) {
-  atomicAnd(&vb[0], (unsigned long long)a);
-  atomicOr(&vb[1], (unsigned long long)o);
-  __syncthreads();
-}
 
 // ---------------------------------------------------------------------------
 // Helpers of the finishing code (k_finish, k_medium): emitting winners held
@@ -1229,73 +1411,6 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict_
 // ---------------------------------------------------------------------------
 constexpr int FIN_UNR = 8;  // float4 loads in flight per thread when re-reading a task
 
-// Bounds L <= U with L at most the ql-th and U at least the qh-th smallest
-// (1 <= ql <= qh <= n) of n distinct composites v[0, n) in LDS that share the
-// first digit `top`, found together over the narrow digits: one histogram
-// per digit while both lie under one prefix, one each after they part.  The
-// lower side stops at a bin whose smallest member is its quantile (L = the
-// lowest composite of that bin), the upper one at a bin whose largest member
-// is (U = the highest composite of that bin), so no sample entry lies in
-// [L, U] beyond the ql-th .. qh-th smallest.
-__device__ __forceinline__ void sample_bounds(const uint64_t* v, uint32_t n, uint32_t ql, uint32_t qh, uint32_t top,
-                                              uint64_t vary, uint64_t same, uint32_t* lh8, uint32_t* wsum,
-                                              uint32_t* res, uint64_t& L, uint64_t& U) {
-  constexpr int shifts[7] = {44, 36, 32, 24, 16, 8, 0};
-  constexpr int widths[7] = {8, 8, 4, 8, 8, 8, 8};
-  uint64_t Pl = top, Ph = top;
-  int sl = 52, sh = 52;
-  bool dl = false, dh = false;
-  for (int q = 0; q < 7 && !(dl && dh); ++q) {
-    const int s = shifts[q], w = widths[q];
-    const uint32_t nb = 1u << w;
-    const int hs = s + w;
-    if (((vary >> s) & (nb - 1)) == 0) {  // one bin
-      const uint64_t fix = (same >> s) & (nb - 1);
-      if (!dl) {
-        Pl = (Pl << w) | fix;
-        sl = s;
-      }
-      if (!dh) {
-        Ph = (Ph << w) | fix;
-        sh = s;
-      }
-      continue;
-    }
-    const bool shared = !dl && !dh && Pl == Ph;
-    uint32_t c[1] = {0u};
-    for (int side = 0; side < 2; ++side) {
-      if (side == 0 ? dl : dh) continue;
-      const uint64_t P = side == 0 ? Pl : Ph;
-      if (!(shared && side == 1)) {  // (the shared histogram serves both)
-        for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) lh8[i] = 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-          const uint64_t x = v[i];
-          if ((x >> hs) == P) hist_add(lh8, (uint32_t)(x >> s) & (nb - 1));
-        }
-        __syncthreads();
-        c[0] = (threadIdx.x < nb) ? lh8[threadIdx.x] : 0u;
-      }
-      block_find_bin<1>(c, side == 0 ? ql : qh, wsum, res);
-      const uint32_t b = res[0], before = res[1], cnt = res[2];
-      __syncthreads();  // res and lh8 are rewritten next
-      if (side == 0) {
-        Pl = (Pl << w) | b;
-        sl = s;
-        ql -= before;
-        dl = ql == 1;
-      } else {
-        Ph = (Ph << w) | b;
-        sh = s;
-        qh -= before;
-        dh = qh == cnt;
-      }
-    }
-  }
-  L = Pl << sl;
-  U = ((Ph + 1ull) << sh) - 1ull;  // (wraps to ~0 at the top)
-}
-
 // LDS of a finishing block beside its dynamic entry array.
 struct FinShared {
   uint32_t lh8[256];
@@ -1348,11 +1463,14 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   const bool reread_done = !c.spill_all && big;
   const bool listed = reread_done || (split && big && split_fits(c, d.len));
   const uint64_t* __restrict__ lst = region + (reread_done ? 0u : n_sp);
-  const uint32_t nlst = fin ? nwin + ncand : (reread_done ? n_sp : ncand);
+  // (a spill need not hold the whole candidate bin: k_split lists the
+  // candidates it holds, list_ctr of them)
+  const uint32_t nlisted = c.list_ctr < ncand ? c.list_ctr : ncand;
+  const uint32_t nlst = reread_done ? n_sp : (fin ? nwin : 0u) + nlisted;
   const bool spilled = c.spill_all && !listed;
   // Candidates of non-fin tasks go to LDS when they fit, else (a re-read
-  // here) to the start of the spill region.
-  const uint32_t ncl = listed ? nlst : ncand;  // candidates of a non-fin task
+  // here) to the start of the spill region.  ncl: at most that many.
+  const uint32_t ncl = listed ? nlst : (spilled ? (ncand < n_sp ? ncand : n_sp) : ncand);
   const bool cand_lds = fin || ncl <= cap;
   if (threadIdx.x == 0) {
     wctr = 0;
@@ -1434,7 +1552,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     __syncthreads();
     block_and_or(a, o, vb);
     const uint32_t cs = cctr < cap ? cctr : cap;
-    const float mu = (float)krem * (float)cs / (float)ncl;
+    const float mu = (float)krem * (float)ns / (float)nss;  // (sampling rate ns / nss)
     const float dev = 4.0f * sqrtf(mu) + 4.0f;
     const int64_t qlo = (int64_t)floorf(mu - dev), qhi = (int64_t)ceilf(mu + dev);
     const uint64_t sv = (uint64_t)(vb[0] ^ vb[1]), ss = (uint64_t)vb[0];
@@ -1577,16 +1695,18 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     bound_by_sample();
   }
   __syncthreads();
+  // candidates gathered: the list's, or those met in the spill or re-read
+  const uint32_t nca = listed ? nlst - (fin ? nwin : 0u) : (cctr < ncand ? cctr : ncand);
   if (exact) block_and_or(a, o, vb);
   const uint64_t vary = (uint64_t)(vb[0] ^ vb[1]), same = (uint64_t)vb[0];
   uint64_t P = bc;
   int s = 52;
   if (fin) {
-    if (krem > 0 && krem < ncand)
-      s = select_digits<true>(LdsSrc{v + nwin}, ncand, 52, P, krem, vary, same, lh8, wsum, res);
+    if (krem > 0 && krem < nca)
+      s = select_digits<true>(LdsSrc{v + nwin}, nca, 52, P, krem, vary, same, lh8, wsum, res);
     uint64_t* sel = v + cap;
     for (uint32_t i = threadIdx.x; i < nwin; i += BLOCK) sel[i] = v[i];
-    for (uint32_t i = threadIdx.x; i < ncand; i += BLOCK) {
+    for (uint32_t i = threadIdx.x; i < nca; i += BLOCK) {
       const uint64_t x = v[nwin + i];
       const bool tk = (x >> s) <= P;
       const uint64_t tm = ballot(tk);
@@ -1603,9 +1723,9 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     // the candidates: in LDS, listed, in the spill region after a re-read
     // here, or (after a misleading sample) in its source
     const uint64_t* __restrict__ src = sampled ? ssrc : (listed ? lst : region);
-    const uint32_t nsrc = sampled ? nss : ncl;
+    const uint32_t nsrc = sampled ? nss : nca;
     const bool in_lds = cand_lds && !sampled;
-    if (krem < ncl) {
+    if (krem < nsrc) {
       if (in_lds) s = select_digits<true>(LdsSrc{v}, nsrc, 52, P, krem, vary, same, lh8, wsum, res);
       else s = select_digits<false>(GlobalSrc{src}, nsrc, 52, P, krem, vary, same, (uint32_t*)v, wsum, res);
     }
@@ -1856,12 +1976,14 @@ __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ s
 }
 
 // Explicit instantiations used by the host.
-#define RADIK_INST(U)                                                                                      \
-  template __global__ void k_hist<U, true>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,     \
-                                           TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                      \
-  template __global__ void k_hist<U, false>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,    \
-                                            TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                     \
-  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, const int64_t*, TaskCtl*,         \
+#define RADIK_INST(U)                                                                                          \
+  template __global__ void k_hist<U, true, true>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,   \
+                                                 TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                   \
+  template __global__ void k_hist<U, true, false>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,  \
+                                                  TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                  \
+  template __global__ void k_hist<U, false, false>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*, \
+                                                   TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                 \
+  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, const int64_t*, TaskCtl*,             \
                                         uint64_t*, const uint32_t*, int64_t);
 RADIK_INST(1)
 RADIK_INST(2)

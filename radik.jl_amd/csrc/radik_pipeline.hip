@@ -73,7 +73,7 @@ int dev_facts(DevFacts** out) {
     }
     f.lds_per_block = budget;
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_hist<4, true>, BLOCK, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_hist<4, true, true>, BLOCK, 0) != hipSuccess ||
         occ < 1) {
       (void)hipGetLastError();
       occ = 4;
@@ -328,7 +328,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   if (rc) return rc;
   int64_t il = 0, is = 0, im = 0, base = 0, mbase = tot, groups = 0, max_m = 0, runs = 0, msum = 0;
   int64_t max_m_med = 0;
-  bool big = false;
+  bool big = false, pos_bounds = false;
   // pass 0: predicted long tasks; pass 1: the others
   for (int pass = 0; pass < 2; ++pass)
   for (int64_t t = 0; t < n; ++t) {
@@ -370,6 +370,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
       p.h_large[il++] = d;
       base += ts.len;
       groups += pred ? sample_groups(ts.len, ts.k) : 0;
+      if (pred && sample_groups(ts.len, ts.k) <= PRED_MAX_G) pos_bounds = true;
     }
   }
   p.h_bases[il] = base;
@@ -377,6 +378,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.h_cbases[il] = runs;
   p.h_mbases[il] = msum;
   p.has_big = big;
+  p.pos_bounds = pos_bounds;
   p.n_med = nm;
   p.max_m_med = (uint32_t)max_m_med;
   // Launch shapes of the streaming kernels, pass A's chunk table and the
@@ -513,8 +515,12 @@ int upload(Pipeline& p, hipStream_t s) {
 template <int UNR>
 static void launch_stream_kernels_hist(const Pipeline& p, uint32_t fin_cap, hipStream_t s) {
   if (p.grid_h <= 0) return;
-  hipLaunchKernelGGL((k_hist<UNR, true>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                     (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
+  if (p.pos_bounds)
+    hipLaunchKernelGGL((k_hist<UNR, true, true>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
+                       (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
+  else
+    hipLaunchKernelGGL((k_hist<UNR, true, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
+                       (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
 }
 
 // Pass B over the whole element space (pass A's chunks, one per block);
@@ -716,7 +722,7 @@ int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel
                      p.d_ctl);
   RADIK_HIP(hipGetLastError());
   if (p.grid_h > 0) {
-    hipLaunchKernelGGL((k_hist<2, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
+    hipLaunchKernelGGL((k_hist<2, false, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
                        (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, 0u);
     RADIK_HIP(hipGetLastError());
   }
