@@ -393,43 +393,37 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
     pp = 0xFFFFFFFFu;
     if (cnt > PRED_MIN && cnt <= PRED_CAP && G <= PRED_MAX_G) {
       // the sampled elements of digit b as composites (the groups of k_sample,
-      // 32 groups in flight at a time), placed by block scans
+      // U groups in flight at a time)
       const float sc = task_scaler(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
       const RADIK_AS1 float* __restrict__ in = glb(d.in);
       uint64_t a = ~0ull, o = 0ull;
-      uint32_t base = 0;  // entries placed by the earlier halves
-      for (int h = 0; h < PRED_MAX_G; h += 32) {
-        if (h >= G) break;
-        float xv[32];
+      constexpr int U = 16;
+      for (int64_t j0 = 0; j0 < G; j0 += U) {
+        float xv[U];
+        uint32_t ps[U];
 #pragma unroll
-        for (int u = 0; u < 32; ++u) {
-          const int64_t i = (int64_t)(h + u) * d.len / G + threadIdx.x;
-          xv[u] = (h + u < G && i < d.len) ? in[i] : 0.0f;
+        for (int u = 0; u < U; ++u) {
+          const int64_t i = (j0 + u) * d.len / G + threadIdx.x;
+          const bool ok = j0 + u < G && i < d.len;
+          xv[u] = ok ? in[i] : 0.0f;
+          ps[u] = ok ? (uint32_t)i : 0xFFFFFFFFu;
         }
-        uint32_t hm = 0;
 #pragma unroll
-        for (int u = 0; u < 32; ++u) {
-          const int64_t i = (int64_t)(h + u) * d.len / G + threadIdx.x;
-          if (h + u < G && i < d.len && (rank_key(xv[u], sc, xm) >> 20) == b) hm |= 1u << u;
-        }
-        const uint32_t mine = (uint32_t)__builtin_popcount(hm);
-        const uint32_t incl = block_incl_scan(mine, wsum);
-        if (threadIdx.x == BLOCK - 1) nsel = base + incl;
-        uint32_t slot = base + incl - mine;
-#pragma unroll
-        for (int u = 0; u < 32; ++u) {
-          if ((hm >> u) & 1u) {
-            const uint32_t pos = (uint32_t)((int64_t)(h + u) * d.len / G + threadIdx.x);
-            const uint64_t x = ((uint64_t)rank_key(xv[u], sc, xm) << 32) | pos;
-            if (slot < PRED_CAP) sv[slot] = x;
-            ++slot;
-            a &= x;
-            o |= x;
+        for (int u = 0; u < U; ++u) {
+          const uint32_t rk = rank_key(xv[u], sc, xm);
+          const bool hit = ps[u] != 0xFFFFFFFFu && (rk >> 20) == b;
+          const uint64_t hm = ballot(hit);
+          if (hm) {
+            const uint32_t slot = wave_append(hit, hm, &nsel);
+            if (hit && slot < PRED_CAP) {
+              const uint64_t x = ((uint64_t)rk << 32) | ps[u];
+              sv[slot] = x;
+              a &= x;
+              o |= x;
+            }
           }
         }
-        __syncthreads();
-        base = nsel;
       }
       __syncthreads();
       block_and_or(a, o, vb);
@@ -623,29 +617,18 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
 #pragma unroll
     for (int u = 0; u < UNR; ++u) r[u] = __builtin_nontemporal_load(pv + v + (int64_t)u * BLOCK);
     uint32_t hit = 0;
+    const uint32_t p0 = (uint32_t)(a + (v << 2));  // position of this lane's first element (POS)
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t rk = rk_of<SCALE>(r[u][j], sc, xm);
         atomicAdd(&lh[rk >> 20], 1u);
-        if (SPILL && rk <= rp) hit |= 1u << (4 * u + j);
+        if (SPILL && spills(rk, p0 + (uint32_t)(u * BLOCK * 4 + j))) hit |= 1u << (4 * u + j);
       }
     }
     if (SPILL && ballot(hit != 0)) {
-      // A rank equal to rp is a hit only up to position pp (checked here, in
-      // iterations with hits only, so the common path keeps its one compare).
-      bool any = true;
-      if (POS) {
-        const uint32_t p0 = (uint32_t)(a + (v << 2));  // position of this lane's first element
-#pragma unroll
-        for (int b = 0; b < 4 * UNR; ++b)
-          if (((hit >> b) & 1u) && rk_of<SCALE>(r[b >> 2][b & 3], sc, xm) == rp &&
-              p0 + (uint32_t)((b >> 2) * BLOCK * 4 + (b & 3)) > pp)
-            hit &= ~(1u << b);
-        any = ballot(hit != 0) != 0;
-      }
-      if (any) {
+      {
         uint64_t mk[4 * UNR];
         uint32_t total = 0;
 #pragma unroll
