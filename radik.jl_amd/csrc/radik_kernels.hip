@@ -275,6 +275,47 @@ __device__ __forceinline__ void sample_bounds(const uint64_t* v, uint32_t n, uin
   U = ((Ph + 1ull) << sh) - 1ull;  // (wraps to ~0 at the top)
 }
 
+// An upper bound U >= the q-th smallest (1 <= q <= n) of n distinct
+// composites v[0, n) in LDS that share the first digit `top`: the narrow
+// digits of the q-th are followed until a bin whose largest member it is; U
+// is the highest composite of that bin (sample_bounds' upper side alone).
+__device__ __forceinline__ uint64_t sample_upper(const uint64_t* v, uint32_t n, uint32_t q, uint32_t top,
+                                                 uint64_t vary, uint64_t same, uint32_t* lh8, uint32_t* wsum,
+                                                 uint32_t* res) {
+  constexpr int shifts[7] = {44, 36, 32, 24, 16, 8, 0};
+  constexpr int widths[7] = {8, 8, 4, 8, 8, 8, 8};
+  uint64_t P = top;
+  int sh = 52;
+  for (int d = 0; d < 7; ++d) {
+    const int s = shifts[d], w = widths[d];
+    const uint32_t nb = 1u << w;
+    const int hs = s + w;
+    if (((vary >> s) & (nb - 1)) == 0) {  // one bin
+      P = (P << w) | ((same >> s) & (nb - 1));
+      sh = s;
+      continue;
+    }
+    for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) lh8[i] = 0;
+    __syncthreads();
+    const uint64_t Pc = P;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+      const uint64_t x = v[i];
+      if ((x >> hs) == Pc) hist_add(lh8, (uint32_t)(x >> s) & (nb - 1));
+    }
+    __syncthreads();
+    uint32_t c[1];
+    c[0] = (threadIdx.x < nb) ? lh8[threadIdx.x] : 0u;
+    block_find_bin<1>(c, q, wsum, res);
+    const uint32_t b = res[0], before = res[1], cnt = res[2];
+    __syncthreads();  // res and lh8 are rewritten next
+    P = (P << w) | b;
+    sh = s;
+    q -= before;
+    if (q == cnt) break;
+  }
+  return ((P + 1ull) << sh) - 1ull;  // (wraps to ~0 at the top)
+}
+
 // ---------------------------------------------------------------------------
 // Sampling: a predicted tasks owns G groups of SAMPLE_GROUP consecutive
 // elements spread evenly over it (group j starts at j * len / G).  Blocks take
@@ -429,8 +470,7 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
       block_and_or(a, o, vb);
       if (nsel == cnt && need > before && need - before <= cnt) {
         const uint32_t qi = need - before;  // the q-th smallest is the qi-th of digit b
-        uint64_t L, Ub;
-        sample_bounds(sv, cnt, qi, qi, b, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh8, wsum, res, L, Ub);
+        const uint64_t Ub = sample_upper(sv, cnt, qi, b, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh8, wsum, res);
         rp = (uint32_t)(Ub >> 32);
         pp = (uint32_t)Ub;
         if (pp != 0xFFFFFFFFu) {  // up to the position before the next group's start
