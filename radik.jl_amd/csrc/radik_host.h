@@ -45,6 +45,8 @@ constexpr size_t FINISH_STATIC_LDS = 2048;
 // sampled, m <= MED_CAP) run whole in one k_medium block beside the passes.
 constexpr int64_t MED_LEN = (int64_t)1 << 16;
 constexpr uint32_t MED_CAP = 2048;
+// Medium-task blocks resident per CU at most (by their LDS request).
+constexpr int MED_MAX_PER_CU = 5;
 
 // Launch-level options.
 struct LaunchOpts {

@@ -540,7 +540,10 @@ static int launch_short_and_medium(const Pipeline& p, const DevFacts& f, const L
     RADIK_HIP(hipGetLastError());
   }
   if (p.n_med) {
-    const size_t lds = ((size_t)MED_CAP + std::min<uint32_t>(p.max_m_med, MED_CAP)) * 8;
+    // (at least 1/(MED_MAX_PER_CU + 1) of a CU's LDS per block, so a CU keeps
+    // room for the sampling and prediction kernels beside the medium tasks)
+    const size_t lds = std::max<size_t>(((size_t)MED_CAP + std::min<uint32_t>(p.max_m_med, MED_CAP)) * 8,
+                                        f.lds_per_block / (MED_MAX_PER_CU + 1) + 256);
     const dim3 g((unsigned)p.n_med), b(BLOCK);
     if (p.max_m_med <= (uint32_t)BLOCK) hipLaunchKernelGGL(k_medium<1>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
     else if (p.max_m_med <= 2u * BLOCK) hipLaunchKernelGGL(k_medium<2>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
