@@ -340,6 +340,38 @@ def test_heavy_ties_reread_tasks():
         assert_same(run_ws(x, [n], k, O.LARGEST), oracle_topk(x, [n], k, O.LARGEST), f"unsampled ties k{k}")
 
 
+def test_uneven_batch_with_duplicates():
+    # The uneven configuration's data at reduced scale (200 tasks of 2^16..2^20
+    # elements, a quarter with 30 % of their entries on 8 fixed values, +-0 /
+    # subnormal / +-Inf specials) through the batched plan: exact ties across
+    # the threshold get a spill bound with a position from the sample.
+    rng = np.random.default_rng(2020)
+    T = 200
+    lens = np.round(2.0 ** (16 + 4 * rng.random(T))).astype(np.int64)
+    sigma = (10.0 ** (-3 + 6 * rng.random(T))).astype(np.float32)
+    dup = rng.random(T) < 0.25
+    x = rng.standard_normal(int(lens.sum()), dtype=np.float32)
+    tid = np.repeat(np.arange(T), lens)
+    x *= sigma[tid]
+    fixed = np.arange(1, 9, dtype=np.float32) * np.float32(0.5)
+    m = dup[tid] & (rng.random(x.size, dtype=np.float32) < 0.3)
+    x[m] = fixed[rng.integers(0, 8, int(m.sum()))] * sigma[tid[m]]
+    sp = np.nonzero(rng.random(x.size) < 1e-4)[0]
+    x[sp] = np.array([0.0, -0.0, 1.4e-45, -1.4e-45, np.inf, -np.inf], np.float32)[rng.integers(0, 6, sp.size)]
+    d = DeviceArray.from_host(x)
+    for k, largest, rev in ((256, True, False), (1, True, True), (1000, True, False), (256, False, True)):
+        flags = (O.LARGEST if largest else 0) | (0 if rev else O.ASCEND)
+        say("uneven dups", k, largest, rev)
+        p = radik.Plan(list(lens), k, largest=largest, rev=rev)
+        vo = DeviceArray.zeros((k, T), np.float32)
+        io = DeviceArray.zeros((k, T), np.int32)
+        for rep in range(2):
+            p.execute(vo, io, d)
+            radik.synchronize()
+            assert_same((vo.to_host(), io.to_host()), oracle_topk(x, list(lens), k, flags),
+                        f"uneven dups k{k} largest{largest} rev{rev} rep{rep}")
+
+
 def test_large_k_multi_block_sort():
     # k above 4096 sorts in several blocks (runs + ranks); ties and specials
     rng = np.random.default_rng(16384)
