@@ -9,11 +9,12 @@
 //
 //   k_sample   first-digit histogram of an evenly spread ~1/64 sample of each
 //              long task
-//   k_predict  per task: a first-digit bound b_pred that the task's m best
-//              keys stay under with high probability
+//   k_predict  per task: a bound on composites (a first digit, narrowed to a
+//              rank and position for heavy digits) that the task's m best
+//              stay under with high probability
 //   k_hist     pass A, the one full read: exact 4096-bin histogram per task
 //              (LDS ds_add, one global atomic per non-zero bin per chunk) and,
-//              in the same read, a spill of every key with digit <= b_pred
+//              in the same read, a spill of every composite under the bound
 //              (per-wave LDS staging); tasks inside one chunk are selected at
 //              once from the LDS histogram               ~ count_bin_ex_kernel!
 //   k_select   (tasks over 2^20 elements) block scan of the 4096 bins ->
