@@ -288,6 +288,16 @@ def test_prediction_miss_forces_reread():
         x[st:st + 256] = 100.0 + np.arange(j * 256, j * 256 + 256, dtype=np.float32)
     say("prediction miss")
     assert_same(run_ws(x, [n], k, O.LARGEST | O.ASCEND), oracle_topk(x, [n], k, O.LARGEST | O.ASCEND), "miss")
+    # An exact tie seen only by the sample (4 entries at the start of each
+    # sample group): the bound narrowed to a position covers 16 of the 256
+    # ties, the spill falls short of m and the task is re-read.
+    x = np.random.default_rng(4).standard_normal(n).astype(np.float32)
+    for j in range(G):
+        st = j * n // G
+        x[st:st + 4] = np.float32(5.0)
+    say("position-bound miss")
+    for k in (256, 200):
+        assert_same(run_ws(x, [n], k, O.LARGEST), oracle_topk(x, [n], k, O.LARGEST), f"position miss k{k}")
 
 
 def test_heavy_ties_at_the_threshold():
