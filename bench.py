@@ -55,19 +55,23 @@ def uneven_batch(T=4096, seed=4096):
     return x, lens
 
 
-def dominant_roofline(plan, phase_ms, steps, N, out_pairs, idx_bytes):
-    """Roofline of the slower of the two streaming kernels: pass A (k_hist)
-    reads every element once (4 B/element); k_compact re-reads the long tasks
-    whose spill does not hold their selection and writes the winner pairs."""
+def dominant_roofline(phase_ms, steps, info, out_pairs, idx_bytes, workload):
+    """Roofline of the dominant kernel, pass A (k_hist): it reads every
+    element of the long tasks once (4 B/element; the short and medium tasks
+    are read by k_small / k_medium on the side stream).  When the re-read
+    pass k_compact takes longer it is the dominant one instead (4 B/element
+    of the long tasks plus the winner pairs it writes)."""
+    n_long = info["long_elements"]
     hist_ms = phase_ms["hist"] / steps
     comp_ms = phase_ms["compact"] / steps
     if comp_ms >= hist_ms:
-        name, t_ms, byts = "k_compact", comp_ms, 4.0 * N + (4 + idx_bytes) * out_pairs
+        name, t_ms, byts = "k_compact", comp_ms, 4.0 * n_long + (4 + idx_bytes) * out_pairs
     else:
-        name, t_ms, byts = "k_hist", hist_ms, 4.0 * N
+        name, t_ms, byts = "k_hist", hist_ms, 4.0 * n_long
     achieved = byts / (t_ms * 1e-3)
     return name, {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                  "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic_from_profiles(name)}
+                  "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                  "traffic": traffic_from_profiles(name, workload)}, byts
 
 
 def path_fracs(ms, N, k, T, idx_bytes=4):
@@ -82,16 +86,30 @@ def path_fracs(ms, N, k, T, idx_bytes=4):
 
 
 def traffic_from_profiles(kernel, workload="uneven"):
-    """HBM bytes per launch of `kernel` on `workload`, from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (scaled by the copy-kernel
-    calibration), if any."""
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE passes (each scaled by what the counter
+    reads for the plain copy kernel), or None.  profiles/traffic.json names
+    the library source hash it was measured on; a figure measured on other
+    kernel sources is not reported."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(workload, {}).get(kernel)
     except (OSError, ValueError):
         return None
+    if t.get("kernels_sha256_16") != kernels_hash():
+        return None
+    return t.get(workload, {}).get(kernel)
+
+
+def kernels_hash():
+    """First 16 hex digits of the SHA-256 of the kernel sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("radik_kernels.hip", "radik_pipeline.hip", "radik_device.h", "radik_host.h"):
+        with open(os.path.join(ROOT, "radik.jl_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def time_plan(radik, plan, vo, io, d_x, steps, warmup):
@@ -121,28 +139,56 @@ def phase_times(radik, plan, vo, io, d_x, steps, d_i=None):
     return ms, n
 
 
-def cpu_baseline(x, lens, k, flags, max_tasks=256):
-    """Oracle's restatement of the reference schedule on host cores, on the
-    first max_tasks tasks of the workload (threads = min(16, cores))."""
-    import oracle as O
-    sub = lens[:max_tasks]
-    n = int(sub.sum())
-    xs = x[:n]
-    threads = max(1, min(16, os.cpu_count() or 1))
-    O.topk_schedule(xs[:100000], [100000], k, flags, threads=threads)  # warm
+def _median_time(fn, budget_s, min_runs=5, max_runs=2000):
+    fn()
+    fn()
+    fn()  # warm-up
     times = []
-    t_end = time.perf_counter() + 20.0
-    while len(times) < 3 or (time.perf_counter() < t_end and len(times) < 10):
+    t_end = time.perf_counter() + budget_s
+    while len(times) < min_runs or (time.perf_counter() < t_end and len(times) < max_runs):
         t0 = time.perf_counter()
-        O.topk_schedule(xs, sub, k, flags, threads=threads)
+        fn()
         times.append(time.perf_counter() - t0)
-        if time.perf_counter() > t_end:
-            break
-    t = float(np.median(times))
-    return {"value": round(n / t / 1e9, 4), "unit": "Gelem/s", "cores": threads, "kind": "port",
-            "sample": f"first {len(sub)} tasks of the uneven batch ({n} elements), k={k}, "
-                      f"reference 3-pass schedule restated in C (oracle/radik_oracle.c), OpenMP over tasks, "
-                      f"median of {len(times)} runs"}
+    return float(np.median(times)), len(times)
+
+
+def host_threads():
+    """Host threads for the CPU baseline: the CPU share the environment gives
+    this process (OMP_NUM_THREADS when set), else every core."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return os.cpu_count() or 1
+
+
+def cpu_baseline(x4, lens4, k4, flags):
+    """The reference's 3-pass schedule restated in C (oracle/radik_oracle.c,
+    OpenMP) on the host: the headline line is the plain single-task
+    configuration (n = 1e6 N(0,1), seed 42, k = 100, largest, automatic
+    indices) on every thread the process may use, the element range split
+    across threads.  Beside it: the textbook line (quickselect + sort of
+    (key, position) pairs, one thread: nth_element + sort) on the same task,
+    and the schedule over the first 256 tasks of the uneven batch (threads
+    over tasks)."""
+    import oracle as O
+    threads = host_threads()
+    x1 = np.random.default_rng(42).standard_normal(1_000_000, dtype=np.float32)
+    t_sched, r1 = _median_time(lambda: O.topk_schedule(x1, [x1.size], 100, O.LARGEST | O.ASCEND,
+                                                          threads=threads), 8.0)
+    t_text, r2 = _median_time(lambda: O.topk_exact(x1, [x1.size], 100, O.LARGEST | O.ASCEND, threads=1), 4.0)
+    sub = lens4[:256]
+    n = int(sub.sum())
+    xs = x4[:n]
+    t_c4, r3 = _median_time(lambda: O.topk_schedule(xs, sub, k4, flags, threads=threads), 8.0, min_runs=3,
+                            max_runs=10)
+    return {"value": round(x1.size / t_sched / 1e9, 4), "unit": "Gelem/s", "cores": threads, "kind": "port",
+            "sample": (f"n=1e6 N(0,1) seed 42, k=100, largest, rev=false (the plain single-task configuration): "
+                       f"reference 3-pass schedule restated in C (oracle/radik_oracle.c), OpenMP over the "
+                       f"element range, {threads} threads, median of {r1} runs, {t_sched * 1e3:.3f} ms"),
+            "textbook_nth_element_sort": {"value": round(x1.size / t_text / 1e9, 4), "unit": "Gelem/s",
+                                          "cores": 1, "ms": round(t_text * 1e3, 3), "runs": r2},
+            "uneven_first_256_tasks": {"value": round(n / t_c4 / 1e9, 4), "unit": "Gelem/s", "cores": threads,
+                                       "elements": n, "runs": r3}}
 
 
 def secondary(radik, O, warmup, steps):
@@ -157,7 +203,20 @@ def secondary(radik, O, warmup, steps):
         vo = DeviceArray.zeros((100, 1000), np.float32)
         io = DeviceArray.zeros((100, 1000), np.int32)
         ms, _ = time_plan(radik, p, vo, io, d, max(steps, 20), warmup)
-        out[name] = {"ms": round(ms, 4), "Gelem_s": round(1e6 / (ms * 1e-3) / 1e9, 2)}
+        # the drop-in topk! into the same pre-allocated buffers (synchronous)
+        t_drop, _ = _median_time(lambda: radik.topk_(vo, io, d, [1000] * 1000, 100), 1.0, max_runs=200)
+        out[name] = {"ms": round(ms, 4), "Gelem_s": round(1e6 / (ms * 1e-3) / 1e9, 2),
+                     "topk_drop_in_ms": round(t_drop * 1e3, 4)}
+    # the plain single task: n = 1e6 N(0,1) (seed 42), k = 100, drop-in topk
+    x1 = np.random.default_rng(42).standard_normal(1_000_000, dtype=np.float32)
+    d1 = DeviceArray.from_host(x1)
+    t1, _ = _median_time(lambda: radik.topk(d1, 100), 1.0, max_runs=500)
+    v1, i1 = radik.topk(d1, 100)
+    e1v, e1i = O.topk_exact(x1, [x1.size], 100, O.LARGEST | O.ASCEND)
+    out["single_1e6_k100_topk"] = {"ms": round(t1 * 1e3, 4), "Gelem_s": round(1e6 / t1 / 1e9, 3),
+                                   "matches_oracle": bool(np.array_equal(v1.to_host().view(np.uint32),
+                                                                         e1v[0].view(np.uint32)) and
+                                                          np.array_equal(i1.to_host(), e1i[0]))}
     # single task 2^28, Int32 custom indices (a permutation), k grid
     n = 1 << 28
     log("secondary: 2^28 single task")
@@ -372,21 +431,30 @@ def main():
     value = N / (ms * 1e-3) / 1e9
     log(f"batched: {ms:.3f} ms/step ({value:.1f} Gelem/s)")
 
-    # parity spot check of the timed output (a few tasks) against the oracle
+    # parity of the whole timed output against the oracle (values and
+    # indices bit-exact); a mismatch voids the measurement
     vh, ih = vo.to_host(), io.to_host()
     off = np.concatenate([[0], np.cumsum(lens)])
-    check_ok = True
-    for t in sorted({0, min(1, T - 1), T // 2, T - 1}):
-        ev, ei = O.topk_exact(x[off[t]:off[t + 1]], [lens[t]], k, O.LARGEST | O.ASCEND)
-        check_ok &= bool(np.array_equal(vh[:, t].view(np.uint32), ev[0].view(np.uint32)) and
-                         np.array_equal(ih[:, t], ei[0]))
+    t_chk = time.perf_counter()
+    ev, ei = O.topk_exact(x, lens, k, O.LARGEST | O.ASCEND, threads=host_threads())
+    bad_v = int(np.count_nonzero(vh.view(np.uint32) != ev.T.view(np.uint32)))
+    bad_i = int(np.count_nonzero(ih.astype(np.int64) != ei.T.astype(np.int64)))
+    check_ok = bad_v == 0 and bad_i == 0
+    del ev, ei
+    log(f"oracle check of all {T} tasks: {'ok' if check_ok else 'MISMATCH'} "
+        f"({bad_v} values, {bad_i} indices differ; {time.perf_counter() - t_chk:.1f}s)")
 
+    info = plan.info()
     phase, cnt = phase_times(radik, plan, vo, io, d_x, args.steps)
-    kname, roof = dominant_roofline(plan, phase, cnt, N, k * T, 4)
+    kname, roof, kbytes = dominant_roofline(phase, cnt, info, k * T, 4, args.workload)
 
     extra = {"phase_ms": {kk: round(v / cnt, 4) for kk, v in phase.items()}, "dominant_kernel": kname,
+             "dominant_kernel_bytes": int(kbytes), "plan_info": info,
              **path_fracs(ms, N, k, T), "wall_ms_per_step": round(wall_ms, 4),
-             "oracle_spot_check": check_ok}
+             "oracle_check": {"tasks": T, "ok": check_ok, "values_differ": bad_v, "indices_differ": bad_i}}
+    if roof["traffic"]:
+        roof_ratio = roof["traffic"] / kbytes
+        extra["dominant_traffic_over_algorithmic"] = round(roof_ratio, 4)
 
     if not args.no_plain:
         # plain form: one drop-in call per task (synchronous, as the reference)
@@ -419,7 +487,7 @@ def main():
 
     line = {
         "metric": "Gelem/s (batched top-k, uneven batch) + % HBM peak",
-        "value": round(value, 3),
+        "value": round(value, 3) if check_ok else None,
         "unit": "Gelem/s",
         "n_gpus": 1,
         "steps": args.steps,

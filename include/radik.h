@@ -106,6 +106,13 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * LDS, small = short and medium tasks) and the number of executes timed. */
 int radik_plan_set_timing(radik_plan *plan, int enable);
 int radik_plan_timing(radik_plan *plan, double *ms8, int64_t *count);
+/* How the last execute staged the plan's tasks (after the first execute):
+ * info[0..8] = long tasks (the streaming passes), their elements (what
+ * pass A reads), medium tasks, their elements, short tasks, their elements,
+ * pass-A chunks (its grid), sample groups, whether some task exceeds 2^20
+ * elements.  n: entries wanted (up to RADIK_PLAN_INFO_LEN). */
+#define RADIK_PLAN_INFO_LEN 9
+int radik_plan_info(radik_plan *plan, int64_t *info, int n);
 int radik_plan_destroy(radik_plan *plan);
 
 /* Serving engine (no reference counterpart): continuous batching of top-k

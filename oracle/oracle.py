@@ -197,3 +197,11 @@ def topk_schedule(val_in, task_lens, k, flags, idx_in=None, idx_dtype=np.int32, 
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+def reorder(v, descending):
+    """Permutation that stably re-sorts a list of values by key (ascending or
+    descending): equal keys keep their relative order.  Turns an oracle
+    output in one order (ties by position) into the other order."""
+    kk = keys(v).astype(np.int64)
+    return np.lexsort((np.arange(v.size), -kk if descending else kk))

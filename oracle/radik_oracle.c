@@ -509,6 +509,144 @@ static void topk_schedule_task(const float *data, int64_t n, int64_t k, uint32_t
     free(cand); free(next); free(sel); free(selv);
 }
 
+/* The same schedule for one long task on several host threads (the CPU
+ * baseline of a single task): the full-input steps -- pass-1 histogram,
+ * pass-1 compaction, filter -- split the task into one contiguous range per
+ * thread; the filter admits threshold-equal elements in position order
+ * (each range's share of the boundary count from a prefix over the ranges),
+ * so the outputs equal topk_schedule_task's. */
+static void topk_schedule_task_par(const float *data, int64_t n, int64_t k, uint32_t flags,
+                                   const void *idx_in, int idx_bytes, float *val_out, void *idx_out, int64_t *st,
+                                   int nthr)
+{
+#ifndef _OPENMP
+    topk_schedule_task(data, n, k, flags, idx_in, idx_bytes, val_out, idx_out, st);
+    (void)nthr;
+#else
+    if (n <= k || nthr <= 1) {
+        topk_schedule_task(data, n, k, flags, idx_in, idx_bytes, val_out, idx_out, st);
+        return;
+    }
+    const int largest = (flags & ORC_LARGEST) != 0;
+    const int ws = (flags & ORC_WITHSCALE) != 0;
+    for (int i = 0; i < ORC_NSTAT; i++) st[i] = 0;
+    const float s = orc_sample_scaler(data[0], ws);
+    int64_t *hist = (int64_t *)calloc((size_t)nthr * 4096, sizeof(int64_t));
+    int64_t *cnt = (int64_t *)calloc((size_t)nthr + 1, sizeof(int64_t));
+    int64_t *cnt2 = (int64_t *)calloc((size_t)nthr + 1, sizeof(int64_t));
+    int64_t res[3];
+#pragma omp parallel num_threads(nthr)
+    {
+        const int t = omp_get_thread_num(), P = omp_get_num_threads();
+        const int64_t a = n * t / P, b = n * (t + 1) / P;
+        int64_t *h = hist + (size_t)t * 4096;
+        for (int64_t i = a; i < b; i++) h[orc_key(orc_scaled(data[i], s, ws)) >> 20]++;
+    }
+    int64_t h1[4096];
+    for (int b = 0; b < 4096; b++) {
+        h1[b] = 0;
+        for (int t = 0; t < nthr; t++) h1[b] += hist[(size_t)t * 4096 + b];
+    }
+    orc_select_bin(h1, 4096, k, largest, res);
+    st[0] = res[0]; st[1] = res[1]; st[2] = res[2];
+    int64_t kk = res[1];
+    const int64_t bin1 = res[0];
+    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(res[2] > 0 ? res[2] : 1));
+    uint32_t *next = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(res[2] > 0 ? res[2] : 1));
+    /* pass-1 compaction: per-range counts, then each range writes at its offset */
+#pragma omp parallel num_threads(nthr)
+    {
+        const int t = omp_get_thread_num(), P = omp_get_num_threads();
+        const int64_t a = n * t / P, b = n * (t + 1) / P;
+        int64_t c = 0;
+        for (int64_t i = a; i < b; i++)
+            if ((int64_t)(orc_key(orc_scaled(data[i], s, ws)) >> 20) + 1 == bin1) c++;
+        cnt[t + 1] = c;
+#pragma omp barrier
+#pragma omp single
+        for (int u = 0; u < P; u++) cnt[u + 1] += cnt[u];
+        int64_t o = cnt[t];
+        for (int64_t i = a; i < b; i++) {
+            const uint32_t key = orc_key(orc_scaled(data[i], s, ws));
+            if ((int64_t)(key >> 20) + 1 == bin1) cand[o++] = key;
+        }
+    }
+    int64_t nc = cnt[nthr];
+    int passes = 1, pass2 = 0;
+    int64_t hh[4096];
+    if (nc > 1) {
+        for (int b = 0; b < 4096; b++) hh[b] = 0;
+        for (int64_t i = 0; i < nc; i++) hh[(cand[i] >> 8) & 0xFFFu]++;
+        orc_select_bin(hh, 4096, kk, largest, res);
+        st[3] = res[0]; st[4] = res[1]; st[5] = res[2];
+        kk = res[1];
+        int64_t nn = 0;
+        for (int64_t i = 0; i < nc; i++)
+            if ((int64_t)((cand[i] >> 8) & 0xFFFu) + 1 == res[0]) next[nn++] = cand[i];
+        uint32_t *tp = cand; cand = next; next = tp; nc = nn;
+        pass2 = 1;
+        passes = 2;
+    }
+    if (nc > 1 && pass2) {
+        for (int b = 0; b < 256; b++) hh[b] = 0;
+        for (int64_t i = 0; i < nc; i++) hh[cand[i] & 0xFFu]++;
+        orc_select_bin(hh, 256, kk, largest, res);
+        st[6] = res[0]; st[7] = res[1]; st[8] = res[2];
+        kk = res[1];
+        int64_t nn = 0;
+        for (int64_t i = 0; i < nc; i++)
+            if ((int64_t)(cand[i] & 0xFFu) + 1 == res[0]) next[nn++] = cand[i];
+        uint32_t *tp = cand; cand = next; next = tp; nc = nn;
+        passes = 3;
+    }
+    const uint32_t kth = largest ? ~cand[0] : cand[0];
+    st[9] = passes;
+    st[10] = kth;
+    /* filter: strictly better elements, plus the first kk threshold-equal ones
+     * in position order */
+    int64_t *sel = (int64_t *)malloc(sizeof(int64_t) * (size_t)k);
+    float *selv = (float *)malloc(sizeof(float) * (size_t)k);
+    for (int t = 0; t <= nthr; t++) { cnt[t] = 0; cnt2[t] = 0; }
+#pragma omp parallel num_threads(nthr)
+    {
+        const int t = omp_get_thread_num(), P = omp_get_num_threads();
+        const int64_t a = n * t / P, b = n * (t + 1) / P;
+        int64_t lt = 0, eq = 0;
+        for (int64_t i = a; i < b; i++) {
+            const uint32_t r = rank_key(data[i], s, flags);
+            lt += r < kth;
+            eq += r == kth;
+        }
+        cnt[t + 1] = lt;
+        cnt2[t + 1] = eq;
+#pragma omp barrier
+#pragma omp single
+        for (int u = 0; u < P; u++) {
+            cnt2[u + 1] += cnt2[u];  /* equal elements before range u + 1 */
+        }
+        /* range t admits equal elements with global equal-rank < kk */
+        const int64_t e0 = cnt2[t];
+        int64_t admit = kk - e0;
+        if (admit < 0) admit = 0;
+        if (admit > eq) admit = eq;
+        cnt[t + 1] += admit;
+#pragma omp barrier
+#pragma omp single
+        for (int u = 0; u < P; u++) cnt[u + 1] += cnt[u];
+        int64_t o = cnt[t], left = admit;
+        for (int64_t i = a; i < b; i++) {
+            const uint32_t r = rank_key(data[i], s, flags);
+            int take = r < kth;
+            if (!take && r == kth && left > 0) { left--; take = 1; }
+            if (take && o < k) { selv[o] = data[i]; sel[o] = i; o++; }
+        }
+    }
+    int64_t c = cnt[nthr] < k ? cnt[nthr] : k;
+    write_sorted(data, sel, c, k, flags, idx_in, idx_bytes, val_out, idx_out);
+    free(sel); free(selv); free(cand); free(next); free(hist); free(cnt); free(cnt2);
+#endif
+}
+
 int orc_topk_schedule(const float *val_in, const void *idx_in, int idx_bytes, const int64_t *task_lens,
                       int64_t num_tasks, int64_t k, uint32_t flags, float *val_out, void *idx_out,
                       int64_t *stats, int threads)
@@ -519,6 +657,15 @@ int orc_topk_schedule(const float *val_in, const void *idx_in, int idx_bytes, co
     for (int64_t t = 0; t < num_tasks; t++) off[t + 1] = off[t] + task_lens[t];
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
+    if (num_tasks == 1 && threads > 1) {
+        /* one task: the threads split its element range */
+        int64_t local[ORC_NSTAT];
+        const void *ii = (flags & ORC_WITHIDXIN) ? idx_in : NULL;
+        topk_schedule_task_par(val_in, task_lens[0], k, flags, ii, idx_bytes, val_out, idx_out, local, threads);
+        if (stats) memcpy(stats, local, sizeof(local));
+        free(off);
+        return 0;
+    }
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
     for (int64_t t = 0; t < num_tasks; t++) {

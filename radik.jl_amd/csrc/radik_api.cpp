@@ -6,6 +6,8 @@
 #include <string.h>
 
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/radik.h"
@@ -28,6 +30,8 @@ struct radik_ws {
   int counter_bytes = 4;
   Pipeline pipe;
   std::vector<TaskSpec> specs;
+  hipStream_t last_stream = nullptr;  // the stream of the last call (its work may still reference the buffers)
+  bool used = false;
 };
 
 struct radik_plan {
@@ -62,6 +66,14 @@ struct radik_plan {
       if (e) (void)hipEventDestroy(e);
   }
 };
+
+// One cached workspace per device for radik_topk (the allocating drop-in).
+constexpr int kMaxDevices = 64;
+struct TopkCache {
+  std::mutex mu;
+  radik_ws* ws = nullptr;
+};
+static TopkCache g_topk_cache[kMaxDevices];
 
 static const uint32_t kFlagMask = RADIK_LARGEST | RADIK_ASCEND | RADIK_WITHSCALE | RADIK_WITHIDXIN | RADIK_WITHPACKING;
 
@@ -133,7 +145,9 @@ int radik_ws_destroy(radik_ws* ws) {
   const int dev = ws->device;
   (void)hipGetDevice(&cur);
   if (cur != dev) (void)hipSetDevice(dev);
-  (void)hipDeviceSynchronize();
+  // Only the stream of the last call can still use the buffers (the side
+  // stream's work joins it); other streams on the device are not waited for.
+  if (ws->used) (void)hipStreamSynchronize(ws->last_stream);
   delete ws;
   if (cur != dev) (void)hipSetDevice(cur);
   return 0;
@@ -196,6 +210,8 @@ int radik_topk_radix_select(radik_ws* ws, float* val_out, void* idx_out, int idx
     sp.flags = tflags;
     off += task_lens[t];
   }
+  ws->last_stream = s;
+  ws->used = true;
   rc = run_tasks(ws->pipe, ws->specs, opts_for(flags, vload_bytes), s, nullptr);
   if (rc) return rc;
   RADIK_HIP(hipStreamSynchronize(s));
@@ -216,15 +232,35 @@ int radik_topk(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, i
   if (indices && n_indices > 0 && n_indices != total)
     return set_error(RADIK_EINDICES, "indices length must equal data length. Got %lld vs %lld",
                      (long long)n_indices, (long long)total);
-  radik_ws* ws = nullptr;
-  rc = radik_ws_create(&ws, maxlen, num_tasks, 4, stream);
-  if (rc) return rc;
   uint32_t flags = RADIK_WITHPACKING | (largest ? RADIK_LARGEST : 0u) | (rev ? 0u : RADIK_ASCEND);
   if (indices && n_indices > 0) flags |= RADIK_WITHIDXIN;
+  // The reference allocates a workspace per call (algorithm.jl:503).  Here
+  // each device keeps one cached workspace for these calls (grown as needed,
+  // held until the process ends); a call that finds it in use by another
+  // thread makes a temporary one.
+  int dev = 0;
+  RADIK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(RADIK_EINVAL, "device ordinal %d out of range", dev);
+  TopkCache& tc = g_topk_cache[dev];
+  std::unique_lock<std::mutex> lk(tc.mu, std::try_to_lock);
+  radik_ws* ws = nullptr;
+  bool temp = !lk.owns_lock();
+  if (!temp && !tc.ws) {
+    rc = radik_ws_create(&tc.ws, maxlen, num_tasks, 4, stream);
+    if (rc) return rc;
+  }
+  if (temp) {
+    rc = radik_ws_create(&ws, maxlen, num_tasks, 4, stream);
+    if (rc) return rc;
+  } else {
+    ws = tc.ws;
+    if (ws->n < maxlen) ws->n = maxlen;
+    if (ws->num_tasks < num_tasks) ws->num_tasks = num_tasks;
+  }
   rc = radik_topk_radix_select(ws, val_out, idx_out, idx_bytes, val_in, indices, task_lens, num_tasks, k, flags,
                                32, stream);
   std::string keep = rc ? std::string(radik_last_error()) : std::string();
-  radik_ws_destroy(ws);
+  if (temp) radik_ws_destroy(ws);
   if (rc) return set_error(rc, "%s", keep.c_str());
   return 0;
 }
@@ -389,6 +425,17 @@ int radik_plan_execute(radik_plan* p, float* val_out, void* idx_out, const float
     p->graph_stream = s;
   }
   RADIK_HIP(hipGraphLaunch(p->exec, s));
+  return 0;
+}
+
+int radik_plan_info(radik_plan* p, int64_t* info, int n) {
+  clear_error();
+  if (!p || (n > 0 && !info)) return set_error(RADIK_EINVAL, "plan or info is NULL");
+  if (!p->staged) return set_error(RADIK_EINVAL, "the plan has not been executed yet");
+  const Pipeline& q = p->pipe;
+  const int64_t v[RADIK_PLAN_INFO_LEN] = {q.n_large, q.total, q.n_med, q.total_med, q.n_small, q.total_small,
+                                          q.n_chunks, q.total_groups, q.has_big ? 1 : 0};
+  for (int i = 0; i < n; ++i) info[i] = i < RADIK_PLAN_INFO_LEN ? v[i] : 0;
   return 0;
 }
 

@@ -106,6 +106,7 @@ struct Pipeline {
   // current content
   int64_t n_large = 0, n_small = 0, total = 0, total_groups = 0;
   int64_t n_med = 0;
+  int64_t total_med = 0, total_small = 0;  // elements of the medium and short tasks
   uint32_t max_m_med = 0;
   bool has_big = false;    // some long task exceeds BIG_LEN (pass B over several blocks)
   bool pos_bounds = false; // some predicted task may get a spill bound with a position (PRED_MAX_G)

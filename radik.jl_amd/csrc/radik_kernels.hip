@@ -59,17 +59,28 @@ __device__ __forceinline__ int find_task(const int64_t* __restrict__ bases, int 
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Visits elements [s, e) of task array p: scalar head up to 16-byte alignment,
-// UNR float4 loads in flight per thread over the body, scalar tail.  f(x, pos).
+// The vector body of a streaming read starts on a 128-byte boundary (after a
+// scalar head of at most 31 elements), so each wave's 1 KiB load covers
+// whole cache lines: with streaming (non-temporal) loads a line shared by
+// two wave loads is fetched from HBM twice (6 % more traffic on a batch of
+// arbitrarily aligned tasks).
+constexpr uint32_t BODY_ALIGN = 32;  // elements of 4 bytes
+
+__device__ __forceinline__ int64_t aligned_head_end(const float* p, int64_t a, int64_t e) {
+  const int64_t mis = (int64_t)(((uintptr_t)(p + a) >> 2) & (BODY_ALIGN - 1));
+  const int64_t h = a + (((int64_t)BODY_ALIGN - mis) & (int64_t)(BODY_ALIGN - 1));
+  return h < e ? h : e;
+}
+
+// Visits elements [s, e) of task array p: scalar head up to 128-byte
+// alignment, UNR float4 loads in flight per thread over the body, scalar
+// tail.  f(x, pos).
 template <int UNR, typename F>
 __device__ __forceinline__ void stream_range(const float* __restrict__ pp, int64_t s, int64_t e, F&& f) {
   const RADIK_AS1 float* __restrict__ p = glb(pp);
   const int64_t tid = threadIdx.x;
   int64_t a = s;
-  // elements until p + a is 16-byte aligned
-  int64_t mis = (int64_t)(((uintptr_t)(pp + a) >> 2) & 3u);
-  int64_t head_end = a + ((4 - mis) & 3);
-  if (head_end > e) head_end = e;
+  const int64_t head_end = aligned_head_end(pp, a, e);
   for (int64_t i = a + tid; i < head_end; i += BLOCK) f(p[i], (uint32_t)i);
   a = head_end;
   const int64_t nvec = (e - a) >> 2;
@@ -638,16 +649,14 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
     atomicAdd(&lh[rk >> 20], 1u);
     if (SPILL) stage_append(st, spills(rk, pos), ((uint64_t)rk << 32) | pos, ctr, write);
   };
-  // Scalar head up to 16-byte alignment and tail, as in stream_range; the
+  // Scalar head up to 128-byte alignment and tail, as in stream_range; the
   // float4 body spills per iteration: the UNR*4 hits of each lane (a bit
   // mask) enter the wave's stage together, placed by one ballot per element
   // slot, so a rare spill costs a few scalar instructions per iteration.
   const int64_t tid = threadIdx.x;
   const RADIK_AS1 float* __restrict__ p = glb(d.in);
   int64_t a = s;
-  const int64_t mis = (int64_t)(((uintptr_t)(d.in + a) >> 2) & 3u);
-  int64_t head_end = a + ((4 - mis) & 3);
-  if (head_end > e) head_end = e;
+  const int64_t head_end = aligned_head_end(d.in, a, e);
   for (int64_t i = a + tid; i < head_end; i += BLOCK) one(p[i], (uint32_t)i);
   a = head_end;
   const int64_t nvec = (e - a) >> 2;
@@ -1490,7 +1499,9 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   // (a spill need not hold the whole candidate bin: k_split lists the
   // candidates it holds, list_ctr of them)
   const uint32_t nlisted = c.list_ctr < ncand ? c.list_ctr : ncand;
-  const uint32_t nlst = reread_done ? n_sp : (fin ? nwin : 0u) + nlisted;
+  // Pass B and k_split both list fin tasks' winners first (nwin of them),
+  // then the candidates; pass B counts its candidates in cand_ctr (n_sp).
+  const uint32_t nlst = (fin ? nwin : 0u) + (reread_done ? (n_sp < ncand ? n_sp : ncand) : nlisted);
   const bool spilled = c.spill_all && !listed;
   // Candidates of non-fin tasks go to LDS when they fit, else (a re-read
   // here) to the start of the spill region.  ncl: at most that many.

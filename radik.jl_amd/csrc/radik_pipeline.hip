@@ -326,7 +326,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   }
   int rc = p.reserve(nl, ns, nm, tot + tot_med, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, im = 0, base = 0, mbase = tot, groups = 0, max_m = 0, runs = 0, msum = 0;
+  int64_t il = 0, is = 0, im = 0, base = 0, mbase = tot, groups = 0, max_m = 0, runs = 0, msum = 0, tot_small = 0;
   int64_t max_m_med = 0;
   bool big = false, pos_bounds = false;
   // pass 0: predicted long tasks; pass 1: the others
@@ -349,6 +349,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     d.spare = 0;
     if (shrt) {
       d.base = 0;
+      tot_small += ts.len;
       p.h_small[is++] = d;
     } else if (med) {
       d.base = mbase;  // spill region behind the long tasks'
@@ -481,6 +482,8 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.n_large = nl;
   p.n_small = ns;
   p.total = tot;
+  p.total_med = tot_med;
+  p.total_small = tot_small;
   p.total_groups = groups;
   p.max_m = (uint32_t)max_m;
   return 0;

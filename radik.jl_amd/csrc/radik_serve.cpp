@@ -91,11 +91,11 @@ static void dispatch_loop(radik_server* s) {
   }
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
+    // (after stop, requests still pending are dispatched as slots free up)
     s->cv_work.wait(lk, [&] {
-      return s->stop || (!s->pending.empty() && s->inflight < (int)s->slots.size());
+      return (s->stop && s->pending.empty()) || (!s->pending.empty() && s->inflight < (int)s->slots.size());
     });
-    if (s->stop && s->pending.empty()) return;
-    if (s->pending.empty()) continue;
+    if (s->pending.empty()) return;  // stopping, nothing left
     Slot* slot = nullptr;
     for (auto& sl : s->slots)
       if (!sl->busy) { slot = sl.get(); break; }

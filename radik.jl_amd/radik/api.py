@@ -78,6 +78,16 @@ def topk_radix_select_(val_out, idx_out, ws, val_in, idx_in, task_lens, k, LARGE
     lens = [int(x) for x in task_lens]
     flags = _flags(LARGEST, ASCEND, WITHSCALE, WITHIDXIN, WITHPACKING)
     idx_bytes = np.dtype(idx_out.dtype).itemsize
+    # the library sees pointers only: sizes are checked here
+    n_in, n_out = sum(lens), int(k) * len(lens)
+    if val_in.size < n_in:
+        raise RadiKError(_lib.RADIK_ESIZE, f"val_in holds {val_in.size} elements, task_lens sum to {n_in}")
+    if WITHIDXIN and idx_in is not None and idx_in.size < n_in:
+        raise RadiKError(_lib.RADIK_ESIZE, f"idx_in holds {idx_in.size} indices, task_lens sum to {n_in}")
+    if val_out.size < n_out or idx_out.size < n_out:
+        raise RadiKError(_lib.RADIK_ESIZE, f"val_out/idx_out must hold k x num_tasks = {n_out} results")
+    if WITHIDXIN and idx_in is not None and np.dtype(idx_in.dtype).itemsize != idx_bytes:
+        raise TypeError("idx_in and idx_out must share their element width")
     _lib.check(_lib.load().radik_topk_radix_select(
         ws.handle, val_out.ptr, idx_out.ptr, idx_bytes, val_in.ptr,
         idx_in.ptr if (idx_in is not None and WITHIDXIN) else None,
@@ -105,6 +115,8 @@ def topk_(val_out, idx_out, data, *args, indices=None, largest=True, rev=False, 
         raise RadiKError(_lib.RADIK_ESIZE, f"val_out must have size ({int(k)}, {T}), got {tuple(val_out.shape)}")
     if tuple(ishape) != (int(k), T):
         raise RadiKError(_lib.RADIK_ESIZE, f"idx_out must have size ({int(k)}, {T}), got {tuple(idx_out.shape)}")
+    if data.size < sum(lens):
+        raise RadiKError(_lib.RADIK_ESIZE, f"data holds {data.size} elements, task_lens sum to {sum(lens)}")
     n_ind = 0 if indices is None else indices.size
     if indices is not None and n_ind > 0:
         if np.dtype(indices.dtype) != np.dtype(idx_out.dtype):

@@ -45,14 +45,38 @@ class Plan:
         return int(sum(self.ks))
 
     def execute(self, val_out, idx_out, val_in, idx_in=None, stream=None):
+        # buffer sizes are checked here: the library only sees pointers
+        n_in = sum(self.task_lens)
+        if val_in.size < n_in:
+            raise _lib.RadiKError(_lib.RADIK_ESIZE, f"val_in holds {val_in.size} elements, the plan reads {n_in}")
+        if (self.flags & _lib.RADIK_WITHIDXIN) and (idx_in is None or idx_in.size < n_in):
+            raise _lib.RadiKError(_lib.RADIK_ESIZE, f"idx_in must hold {n_in} indices")
+        if val_out.size < self.out_size or idx_out.size < self.out_size:
+            raise _lib.RadiKError(_lib.RADIK_ESIZE, f"val_out/idx_out must hold {self.out_size} results")
+        if np.dtype(idx_out.dtype).itemsize != self.idx_dtype.itemsize:
+            raise _lib.RadiKError(_lib.RADIK_EINVAL, f"idx_out must be {self.idx_dtype}")
+        if idx_in is not None and np.dtype(idx_in.dtype).itemsize != self.idx_dtype.itemsize:
+            raise _lib.RadiKError(_lib.RADIK_EINVAL, f"idx_in must be {self.idx_dtype}")
         _lib.check(_lib.load().radik_plan_execute(self.handle, val_out.ptr, idx_out.ptr, val_in.ptr,
                                                   None if idx_in is None else idx_in.ptr,
                                                   None if stream is None else stream.handle))
+
+    def set_graph(self, use_graph):
+        """Replay a captured hipGraph (True) or launch the kernels eagerly."""
+        _lib.check(_lib.load().radik_plan_set_graph(self.handle, 1 if use_graph else 0))
 
     def set_options(self, vload_bytes=0, blocks_per_cu=0, small_cap=0, refine_cap=0, predict=True):
         """Tuning knobs of the launch sequence; results do not depend on them."""
         _lib.check(_lib.load().radik_plan_set_options(self.handle, int(vload_bytes), int(blocks_per_cu),
                                                       int(small_cap), int(refine_cap), 1 if predict else 0))
+
+    def info(self):
+        """How the last execute staged the tasks (radik_plan_info)."""
+        v = (ctypes.c_int64 * 9)()
+        _lib.check(_lib.load().radik_plan_info(self.handle, v, 9))
+        names = ("n_long", "long_elements", "n_medium", "medium_elements", "n_short", "short_elements",
+                 "hist_chunks", "sample_groups", "has_big")
+        return {nm: int(v[i]) for i, nm in enumerate(names)}
 
     def set_timing(self, on):
         _lib.check(_lib.load().radik_plan_set_timing(self.handle, 1 if on else 0))

@@ -411,3 +411,31 @@ def test_large_k_multi_block_sort():
         assert np.array_equal(i[oo:oo + k], ei[0]), (L, k)
         off += L
         oo += k
+
+
+def test_nan_inputs_follow_total_order():
+    # NaN policy: a NaN is ranked by its key (IEEE totalOrder), so +NaN is the
+    # largest and -NaN the smallest value; payload NaNs keep their bits.
+    # Short, medium, long (sampled) tasks; largest, smallest and WITHSCALE
+    # (a NaN first element scales by 0).
+    rng = np.random.default_rng(2718)
+    nans = np.array([0x7FC00000, 0xFFC00000, 0x7F800001, 0xFF800001, 0x7FFFFFFF, 0xFFFFFFFF, 0x7FC12345],
+                    dtype=np.uint32).view(np.float32)
+    lens = [1000, 30000, 1 << 18, 1 << 20, 3000]
+    x = rng.standard_normal(sum(lens)).astype(np.float32)
+    for frac, (a, b) in zip((0.001, 0.01, 0.0001, 0.00002, 0.2),
+                            np.stack([np.cumsum([0] + lens[:-1]), np.cumsum(lens)], 1)):
+        m = np.nonzero(rng.random(b - a) < frac)[0] + a
+        x[m] = nans[rng.integers(0, 7, m.size)]
+    x[lens[0]] = np.float32(np.nan)  # the second task's first element (its WITHSCALE scaler)
+    for k in (1, 100, 2000):
+        for flags in (O.LARGEST | O.ASCEND, O.LARGEST, 0, O.ASCEND, O.LARGEST | O.WITHSCALE):
+            say("nan", k, flags)
+            assert_same(run_ws(x, lens, k, flags), oracle_topk(x, lens, k, flags), f"nan k{k} f{flags}")
+    p = radik.Plan(lens, 100, largest=True, rev=False)
+    d = DeviceArray.from_host(x)
+    vo = DeviceArray.zeros((100, len(lens)), np.float32)
+    io = DeviceArray.zeros((100, len(lens)), np.int32)
+    p.execute(vo, io, d)
+    radik.synchronize()
+    assert_same((vo.to_host(), io.to_host()), oracle_topk(x, lens, 100, O.LARGEST | O.ASCEND), "nan plan")

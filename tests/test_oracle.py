@@ -220,7 +220,7 @@ def _random_case(rng):
     lens = rng.integers(0, 400, T)
     k = int(rng.integers(0, 120))
     flags = int(rng.integers(0, 8))
-    kind = int(rng.integers(0, 4))
+    kind = int(rng.integers(0, 5))
     n = int(lens.sum())
     if kind == 0:
         x = rng.standard_normal(n)
@@ -228,9 +228,25 @@ def _random_case(rng):
         x = np.round(rng.standard_normal(n) * 2) / 2
     elif kind == 2:
         x = _ieee_specials()[rng.integers(0, 14, n)]
-    else:
+    elif kind == 3:
         x = 1.0 + rng.integers(0, 30, n) * 2.0 ** -21
+    else:
+        x = _with_nans(rng, np.round(rng.standard_normal(n) * 4) / 4)
     return np.asarray(x, np.float32), lens, k, flags
+
+
+def _nans():
+    """Quiet and signalling NaNs of both signs, with payloads."""
+    return np.array([0x7FC00000, 0xFFC00000, 0x7F800001, 0xFF800001, 0x7FFFFFFF, 0xFFFFFFFF, 0x7FC12345],
+                    dtype=np.uint32).view(np.float32)
+
+
+def _with_nans(rng, x):
+    x = np.asarray(x, np.float32).copy()
+    if x.size:
+        m = rng.random(x.size) < 0.1
+        x[m] = _nans()[rng.integers(0, 7, int(m.sum()))]
+    return x
 
 
 def test_exact_matches_lexsort():
@@ -241,6 +257,40 @@ def test_exact_matches_lexsort():
         ev, ei = lexsort_topk(x, lens, k, flags)
         assert np.array_equal(v.view(np.uint32), ev.view(np.uint32))
         assert np.array_equal(i, ei)
+
+
+def test_nan_policy_is_ieee_total_order():
+    """NaN policy of the contract: a NaN is ranked by its key, i.e. by IEEE
+    754 totalOrder: -NaN < -Inf < ... < -0 < +0 < ... < +Inf < +NaN, larger
+    payloads further out.  (The reference skips NaN in its first histogram,
+    src/kernels/count_bin.jl:111, but maps it to floatmin/floatmax when
+    scaling, src/utils.jl:69; the two disagree, so the contract fixes one.)"""
+    vals = np.array([0x7FC00000, 0xFFC00000, 0x7F800000, 0xFF800000, 0x00000000, 0x80000000, 0x3F800000,
+                     0xBF800000, 0x7FFFFFFF, 0xFFFFFFFF, 0x7F800001, 0xFF800001], np.uint32).view(np.float32)
+    # totalOrder stated on the bits: negatives by descending magnitude bits,
+    # then positives by ascending magnitude bits
+    u = vals.view(np.uint32).astype(np.int64)
+    tot = np.where(u >> 31 == 1, -(u & 0x7FFFFFFF) - 1, u)
+    order = np.argsort(tot, kind="stable")
+    n = vals.size
+    v, i = O.topk_exact(vals, [n], n, O.ASCEND)          # smallest first, ascending
+    assert np.array_equal(v[0].view(np.uint32), vals[order].view(np.uint32))
+    v, i = O.topk_exact(vals, [n], 3, O.LARGEST)         # 3 largest, descending
+    assert np.array_equal(v[0].view(np.uint32), vals[order[::-1][:3]].view(np.uint32))
+    assert v[0].view(np.uint32)[0] == 0xFFFFFFFF or np.isnan(v[0][0])
+    # WITHSCALE: a NaN first element scales by 0 and NaNs keep their bits
+    x = np.array([np.nan, 1.0, -np.nan, 3.0, 2.0], np.float32)
+    v, i = O.topk_exact(x, [5], 5, O.LARGEST | O.WITHSCALE | O.ASCEND)
+    ev, ei = lexsort_topk(x, [5], 5, O.LARGEST | O.WITHSCALE | O.ASCEND)
+    assert np.array_equal(v.view(np.uint32), ev.view(np.uint32)) and np.array_equal(i, ei)
+    for fl in (O.LARGEST, 0, O.LARGEST | O.WITHSCALE, O.WITHSCALE | O.ASCEND):
+        y = _with_nans(np.random.default_rng(fl), np.random.default_rng(fl + 9).standard_normal(2000))
+        for k in (1, 7, 300):
+            v, i = O.topk_exact(y, [y.size], k, fl, idx_dtype=np.int64)
+            ev, ei = lexsort_topk(y, [y.size], k, fl)
+            assert np.array_equal(v.view(np.uint32), ev.view(np.uint32)) and np.array_equal(i, ei)
+            a = O.topk_schedule(y, [y.size], k, fl)
+            assert np.array_equal(a[0].view(np.uint32), v.view(np.uint32))
 
 
 def test_exact_brute_force_subsets():
@@ -274,6 +324,43 @@ def test_schedule_matches_exact():
         b = O.topk_schedule(x, lens, k, flags)
         assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
         assert np.array_equal(a[1], b[1])
+
+
+def test_reorder_turns_one_output_order_into_the_other():
+    # top-k in selection order re-sorted by key equals the oracle's other
+    # output order (the 2^28 parity test derives its expectations this way)
+    rng = np.random.default_rng(5)
+    x = (np.round(rng.standard_normal(50000) * 64) / 64).astype(np.float32)
+    x[rng.integers(0, x.size, 300)] = np.array([0.0, -0.0, np.inf], np.float32)[rng.integers(0, 3, 300)]
+    for largest in (True, False):
+        sel = O.LARGEST if largest else O.ASCEND
+        sv, si = O.topk_exact(x, [x.size], 3000, sel)
+        other = (O.LARGEST | O.ASCEND) if largest else 0
+        ov, oi = O.topk_exact(x, [x.size], 3000, other)
+        for k in (1, 17, 3000):
+            ev, ei = O.topk_exact(x, [x.size], k, other)
+            o = O.reorder(sv[0][:k], descending=not largest)
+            assert np.array_equal(sv[0][:k][o].view(np.uint32), ev[0].view(np.uint32))
+            assert np.array_equal(si[0][:k][o], ei[0])
+
+
+def test_schedule_single_task_threads_match_serial():
+    # one task on several threads (the C1 CPU baseline): same outputs and
+    # pass statistics as the serial schedule, ties in position order
+    rng = np.random.default_rng(17)
+    cases = [rng.standard_normal(100000), np.round(rng.standard_normal(100000) * 3) / 3,
+             np.full(50000, 2.5), _with_nans(rng, rng.standard_normal(80000)),
+             _ieee_specials()[rng.integers(0, 14, 60000)]]
+    for x in cases:
+        x = np.asarray(x, np.float32)
+        for k in (1, 100, 4000):
+            for flags in (O.LARGEST | O.ASCEND, 0, O.LARGEST | O.WITHSCALE):
+                a = O.topk_schedule(x, [x.size], k, flags, threads=1, stats=True)
+                b = O.topk_schedule(x, [x.size], k, flags, threads=5, stats=True)
+                assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+                assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+                e = O.topk_exact(x, [x.size], k, flags)
+                assert np.array_equal(e[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(e[1], b[1])
 
 
 def test_schedule_pass_counts():
