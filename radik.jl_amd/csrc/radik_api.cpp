@@ -130,9 +130,10 @@ int radik_ws_create(radik_ws** ws, int64_t n, int64_t num_tasks, int counter_byt
   w->counter_bytes = counter_bytes;
   // Pre-size descriptors and histograms for num_tasks tasks; the spill buffer
   // is sized on first use from the call's lengths.
-  const int64_t nl = (n > (int64_t)f->small_cap) ? num_tasks : 0;
-  const int64_t ns = (n <= (int64_t)f->small_cap) ? num_tasks : 0;
-  rc = w->pipe.reserve(nl, ns, 0, 0, (hipStream_t)stream);
+  StageCounts c;
+  c.nl = (n > (int64_t)f->small_cap) ? num_tasks : 0;
+  c.ns = (n <= (int64_t)f->small_cap) ? num_tasks : 0;
+  rc = w->pipe.reserve(c, 0, (hipStream_t)stream);
   if (rc) return rc;
   *ws = w.release();
   return 0;

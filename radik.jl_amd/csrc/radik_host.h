@@ -64,10 +64,23 @@ inline uint32_t eff_refine_cap(const DevFacts& f, const LaunchOpts& o) {
   return o.refine_cap ? (o.refine_cap < f.refine_cap_max ? o.refine_cap : f.refine_cap_max) : f.refine_cap;
 }
 
-// Device + pinned buffers of one pipeline instance.  Grow-only; the
-// histogram is kept all-zero between launches (k_select clears what it reads).
+// Counts that size a launch's descriptor arrays (see Pipeline).
+struct StageCounts {
+  int64_t nl = 0, ns = 0, nm = 0;  // long, short, medium tasks
+  int64_t nch = 0;                 // pass A's chunks
+  int64_t nbmap = 0;               // block-map entries (chunks + sampling blocks)
+  int64_t nsplit = 0;              // k_split blocks
+};
+
+// Device + host buffers of one pipeline instance.  Every array the host
+// builds for a launch (descriptors, prefix sums, chunk table, block maps)
+// lives in one arena, copied to the device with one copy; the control
+// blocks follow it on the device.  Histograms (kept all-zero between
+// launches: the consumer clears what it reads) and spill regions are
+// separate.  Grow-only.  A fixed pipeline (bind_fixed) lives in storage the
+// caller provides and never allocates: a launch that does not fit fails.
 struct Pipeline {
-  // host copies (pinned)
+  // host copies (pinned, or the caller's storage when fixed)
   TaskDesc* h_large = nullptr;
   int64_t* h_bases = nullptr;   // element prefix sums of the long tasks
   int64_t* h_sbases = nullptr;  // sample-group prefix sums of the long tasks
@@ -93,8 +106,12 @@ struct Pipeline {
   uint32_t* d_hist = nullptr;   // exact first-digit histograms, zero between launches
   uint32_t* d_shist = nullptr;  // sample histograms, zero between launches
   uint64_t* d_cand = nullptr;   // spill regions
-  size_t cap_large = 0, cap_small = 0, cap_med = 0, cap_hist = 0, cap_cand = 0, cap_bmap = 0, cap_xmap = 0,
-         cap_cstart = 0;
+  char* h_arena = nullptr;
+  char* d_arena = nullptr;
+  size_t cap_arena = 0;     // bytes of each arena
+  size_t upload_bytes = 0;  // prefix of the arena the launch copies to the device
+  size_t cap_hist = 0, cap_cand = 0;
+  bool fixed = false;
   int64_t n_split = 0;  // k_split blocks (tasks longer than BIG_LEN)
   // Streaming launch shapes fixed at staging (passes A and B share one chunk
   // table, one chunk per block; sampling has equal chunks): the number of
@@ -114,12 +131,17 @@ struct Pipeline {
   int64_t total_runs = 0, total_m = 0;  // multi-block sort sizes
   hipEvent_t upload_done = nullptr;  // pinned buffers reusable after this
   // k_small runs beside the long-task passes on a second stream (fork/join
-  // events, also inside a graph capture).
+  // events, also inside a graph capture); a fixed pipeline has none and runs
+  // everything on the caller's stream.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   ~Pipeline();
-  // total_cand: spill-region entries (long and medium tasks)
-  int reserve(int64_t n_large, int64_t n_small, int64_t n_med, int64_t total_cand, hipStream_t s);
+  // tot_cand: spill-region entries (long and medium tasks)
+  int reserve(const StageCounts& c, int64_t tot_cand, hipStream_t s);
+  // Places the pipeline in caller storage: dev (device, 256-byte aligned)
+  // holds histograms for max_long tasks, max_cand spill entries and the
+  // arena; host holds the host arena.  Nothing is allocated afterwards.
+  int bind_fixed(char* dev, size_t dev_bytes, char* host, size_t host_bytes, int64_t max_long, int64_t max_cand);
 };
 
 // Splits tasks into short ones (k_small) and long ones (streaming passes),

@@ -99,30 +99,13 @@ int dev_facts(DevFacts** out) {
 // Buffers
 // ---------------------------------------------------------------------------
 Pipeline::~Pipeline() {
-  if (h_large) (void)hipHostFree(h_large);
-  if (h_bases) (void)hipHostFree(h_bases);
-  if (h_sbases) (void)hipHostFree(h_sbases);
-  if (d_sbases) (void)hipFree(d_sbases);
-  if (h_cbases) (void)hipHostFree(h_cbases);
-  if (d_cbases) (void)hipFree(d_cbases);
-  if (h_mbases) (void)hipHostFree(h_mbases);
-  if (d_mbases) (void)hipFree(d_mbases);
-  if (h_bmap) (void)hipHostFree(h_bmap);
-  if (d_bmap) (void)hipFree(d_bmap);
-  if (h_cstart) (void)hipHostFree(h_cstart);
-  if (d_cstart) (void)hipFree(d_cstart);
-  if (h_xmap) (void)hipHostFree(h_xmap);
-  if (d_xmap) (void)hipFree(d_xmap);
-  if (d_shist) (void)hipFree(d_shist);
-  if (h_small) (void)hipHostFree(h_small);
-  if (h_med) (void)hipHostFree(h_med);
-  if (d_med) (void)hipFree(d_med);
-  if (d_large) (void)hipFree(d_large);
-  if (d_bases) (void)hipFree(d_bases);
-  if (d_small) (void)hipFree(d_small);
-  if (d_ctl) (void)hipFree(d_ctl);
-  if (d_hist) (void)hipFree(d_hist);
-  if (d_cand) (void)hipFree(d_cand);
+  if (!fixed) {
+    if (h_arena) (void)hipHostFree(h_arena);
+    if (d_arena) (void)hipFree(d_arena);
+    if (d_hist) (void)hipFree(d_hist);
+    if (d_shist) (void)hipFree(d_shist);
+    if (d_cand) (void)hipFree(d_cand);
+  }
   if (upload_done) (void)hipEventDestroy(upload_done);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
@@ -135,101 +118,134 @@ static size_t grow(size_t have, size_t need) {
   return c;
 }
 
-int Pipeline::reserve(int64_t nl, int64_t ns, int64_t nm, int64_t tot, hipStream_t s) {
-  if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
-  if (!side) {  // default priority, like every stream of this library
-    RADIK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    RADIK_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    RADIK_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Offsets of the arena's arrays for counts c: the uploaded part (everything
+// the host builds), then the device-only control blocks.
+struct ArenaLayout {
+  size_t large, bases, sbases, cbases, mbases, cstart, bmap, xmap, small, med, upload, ctl, total;
+};
+
+static ArenaLayout arena_layout(const StageCounts& c) {
+  ArenaLayout L;
+  size_t o = 0;
+  const size_t pre = (size_t)(c.nl + 1) * sizeof(int64_t);
+  L.large = o; o = align256(o + (size_t)c.nl * sizeof(TaskDesc));
+  L.bases = o; o = align256(o + pre);
+  L.sbases = o; o = align256(o + pre);
+  L.cbases = o; o = align256(o + pre);
+  L.mbases = o; o = align256(o + pre);
+  L.cstart = o; o = align256(o + (size_t)(c.nch + 1) * sizeof(int64_t));
+  L.bmap = o; o = align256(o + (size_t)c.nbmap * sizeof(uint32_t));
+  L.xmap = o; o = align256(o + (size_t)c.nsplit * sizeof(uint2));
+  L.small = o; o = align256(o + (size_t)c.ns * sizeof(TaskDesc));
+  L.med = o; o = align256(o + (size_t)c.nm * sizeof(TaskDesc));
+  L.upload = o;
+  L.ctl = o; o = align256(o + (size_t)c.nl * sizeof(TaskCtl));
+  L.total = o;
+  return L;
+}
+
+int Pipeline::bind_fixed(char* dev, size_t dev_bytes, char* host, size_t host_bytes, int64_t max_long,
+                         int64_t max_cand) {
+  const size_t hb = align256((size_t)max_long * HIST_BINS * sizeof(uint32_t));
+  const size_t cb = align256((size_t)max_cand * sizeof(uint64_t));
+  if (!dev || !host || ((uintptr_t)dev & 255u) || 2 * hb + cb >= dev_bytes)
+    return set_error(1, "fixed pipeline storage too small");
+  fixed = true;
+  d_hist = (uint32_t*)dev;
+  d_shist = (uint32_t*)(dev + hb);
+  d_cand = (uint64_t*)(dev + 2 * hb);
+  cap_hist = (size_t)max_long;
+  cap_cand = (size_t)max_cand;
+  d_arena = dev + 2 * hb + cb;
+  h_arena = host;
+  cap_arena = std::min(dev_bytes - 2 * hb - cb, host_bytes);
+  return 0;
+}
+
+int Pipeline::reserve(const StageCounts& c, int64_t tot, hipStream_t s) {
+  if (!fixed) {
+    if (!upload_done) RADIK_HIP(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
+    if (!side) {  // default priority, like every stream of this library
+      RADIK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+      RADIK_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+      RADIK_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
+    // The pinned arena may still feed an in-flight copy.
+    RADIK_HIP(hipEventSynchronize(upload_done));
   }
-  // The pinned arrays may still feed an in-flight copy.
-  RADIK_HIP(hipEventSynchronize(upload_done));
-  if (!h_large || (size_t)nl > cap_large) {  // the prefix arrays always need their sentinel
+  const ArenaLayout L = arena_layout(c);
+  if (L.total > cap_arena || !h_arena) {
+    if (fixed)
+      return set_error(2, "the launch needs %zu bytes of descriptors, the fixed storage holds %zu", L.total,
+                       cap_arena);
     // Device buffers may be in use by queued work on s.
     RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = grow(cap_large, (size_t)nl);
-    if (h_large) RADIK_HIP(hipHostFree(h_large));
-    if (h_bases) RADIK_HIP(hipHostFree(h_bases));
-    if (h_sbases) RADIK_HIP(hipHostFree(h_sbases));
-    if (h_cbases) RADIK_HIP(hipHostFree(h_cbases));
-    if (h_mbases) RADIK_HIP(hipHostFree(h_mbases));
-    if (d_large) RADIK_HIP(hipFree(d_large));
-    if (d_bases) RADIK_HIP(hipFree(d_bases));
-    if (d_sbases) RADIK_HIP(hipFree(d_sbases));
-    if (d_cbases) RADIK_HIP(hipFree(d_cbases));
-    if (d_mbases) RADIK_HIP(hipFree(d_mbases));
-    if (d_ctl) RADIK_HIP(hipFree(d_ctl));
-    h_large = nullptr; h_bases = nullptr; h_sbases = nullptr; h_cbases = nullptr; h_mbases = nullptr;
-    d_large = nullptr; d_bases = nullptr; d_sbases = nullptr; d_cbases = nullptr; d_mbases = nullptr;
-    d_ctl = nullptr;
-    if (hipHostMalloc((void**)&h_large, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_bases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_sbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_cbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_mbases, (c + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&d_cbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc((void**)&d_mbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc((void**)&d_large, c * sizeof(TaskDesc)) != hipSuccess ||
-        hipMalloc((void**)&d_bases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc((void**)&d_sbases, (c + 1) * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc((void**)&d_ctl, c * sizeof(TaskCtl)) != hipSuccess) {
-      cap_large = 0;
-      return set_error(6, "allocation of %zu task descriptors failed", c);
-    }
-    cap_large = c;
+    const size_t cap = grow(cap_arena, std::max<size_t>(L.total, 4096));
+    if (h_arena) RADIK_HIP(hipHostFree(h_arena));
+    if (d_arena) RADIK_HIP(hipFree(d_arena));
+    h_arena = nullptr;
+    d_arena = nullptr;
+    cap_arena = 0;
+    if (hipHostMalloc((void**)&h_arena, cap, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_arena, cap) != hipSuccess)
+      return set_error(6, "allocation of a %zu-byte descriptor arena failed", cap);
+    cap_arena = cap;
   }
-  if ((size_t)nl > cap_hist) {
+  h_large = (TaskDesc*)(h_arena + L.large);
+  h_bases = (int64_t*)(h_arena + L.bases);
+  h_sbases = (int64_t*)(h_arena + L.sbases);
+  h_cbases = (int64_t*)(h_arena + L.cbases);
+  h_mbases = (int64_t*)(h_arena + L.mbases);
+  h_cstart = (int64_t*)(h_arena + L.cstart);
+  h_bmap = (uint32_t*)(h_arena + L.bmap);
+  h_xmap = (uint2*)(h_arena + L.xmap);
+  h_small = (TaskDesc*)(h_arena + L.small);
+  h_med = (TaskDesc*)(h_arena + L.med);
+  d_large = (TaskDesc*)(d_arena + L.large);
+  d_bases = (int64_t*)(d_arena + L.bases);
+  d_sbases = (int64_t*)(d_arena + L.sbases);
+  d_cbases = (int64_t*)(d_arena + L.cbases);
+  d_mbases = (int64_t*)(d_arena + L.mbases);
+  d_cstart = (int64_t*)(d_arena + L.cstart);
+  d_bmap = (uint32_t*)(d_arena + L.bmap);
+  d_xmap = (uint2*)(d_arena + L.xmap);
+  d_small = (TaskDesc*)(d_arena + L.small);
+  d_med = (TaskDesc*)(d_arena + L.med);
+  d_ctl = (TaskCtl*)(d_arena + L.ctl);
+  upload_bytes = L.upload;
+  if ((size_t)c.nl > cap_hist) {
+    if (fixed) return set_error(2, "the launch has %lld long tasks, the fixed storage holds %zu", (long long)c.nl,
+                                cap_hist);
     RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = grow(cap_hist, (size_t)nl);
+    size_t cap = grow(cap_hist, (size_t)c.nl);
     if (d_hist) RADIK_HIP(hipFree(d_hist));
     if (d_shist) RADIK_HIP(hipFree(d_shist));
     d_hist = nullptr;
     d_shist = nullptr;
-    if (hipMalloc((void**)&d_hist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void**)&d_shist, c * HIST_BINS * sizeof(uint32_t)) != hipSuccess) {
-      cap_hist = 0;
-      return set_error(6, "allocation of %zu task histograms failed", c);
-    }
-    RADIK_HIP(hipMemset(d_hist, 0, c * HIST_BINS * sizeof(uint32_t)));
-    RADIK_HIP(hipMemset(d_shist, 0, c * HIST_BINS * sizeof(uint32_t)));
-    cap_hist = c;
-  }
-  if ((size_t)ns > cap_small) {
-    RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = grow(cap_small, (size_t)ns);
-    if (h_small) RADIK_HIP(hipHostFree(h_small));
-    if (d_small) RADIK_HIP(hipFree(d_small));
-    h_small = nullptr; d_small = nullptr;
-    if (hipHostMalloc((void**)&h_small, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&d_small, c * sizeof(TaskDesc)) != hipSuccess) {
-      cap_small = 0;
-      return set_error(6, "allocation of %zu short-task descriptors failed", c);
-    }
-    cap_small = c;
-  }
-  if ((size_t)nm > cap_med) {
-    RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = grow(cap_med, (size_t)nm);
-    if (h_med) RADIK_HIP(hipHostFree(h_med));
-    if (d_med) RADIK_HIP(hipFree(d_med));
-    h_med = nullptr; d_med = nullptr;
-    if (hipHostMalloc((void**)&h_med, c * sizeof(TaskDesc), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&d_med, c * sizeof(TaskDesc)) != hipSuccess) {
-      cap_med = 0;
-      return set_error(6, "allocation of %zu medium-task descriptors failed", c);
-    }
-    cap_med = c;
+    cap_hist = 0;
+    if (hipMalloc((void**)&d_hist, cap * HIST_BINS * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&d_shist, cap * HIST_BINS * sizeof(uint32_t)) != hipSuccess)
+      return set_error(6, "allocation of %zu task histograms failed", cap);
+    RADIK_HIP(hipMemset(d_hist, 0, cap * HIST_BINS * sizeof(uint32_t)));
+    RADIK_HIP(hipMemset(d_shist, 0, cap * HIST_BINS * sizeof(uint32_t)));
+    cap_hist = cap;
   }
   if ((size_t)tot > cap_cand) {
+    if (fixed)
+      return set_error(2, "the launch needs %lld spill entries, the fixed storage holds %zu", (long long)tot,
+                       cap_cand);
     RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = (size_t)tot;
+    // geometric growth (1.5x), so a slowly growing workload does not
+    // reallocate (and synchronise) on every call
+    const size_t cap = std::max<size_t>((size_t)tot, cap_cand + cap_cand / 2);
     if (d_cand) RADIK_HIP(hipFree(d_cand));
     d_cand = nullptr;
-    if (hipMalloc((void**)&d_cand, c * sizeof(uint64_t)) != hipSuccess) {
-      cap_cand = 0;
-      return set_error(6, "allocation of a %zu-entry candidate spill buffer failed", c);
-    }
-    cap_cand = c;
+    cap_cand = 0;
+    if (hipMalloc((void**)&d_cand, cap * sizeof(uint64_t)) != hipSuccess)
+      return set_error(6, "allocation of a %zu-entry candidate spill buffer failed", cap);
+    cap_cand = cap;
   }
   return 0;
 }
@@ -306,28 +322,63 @@ static inline bool is_short(int64_t len, uint32_t small_cap) {
   return small_cap > 0 && len <= (int64_t)small_cap;
 }
 
+// Pass A's table: with blocks_per_cu up to the resident pass-A blocks per
+// CU, equal chunks for blocks_per_cu blocks per CU (one round when they all
+// fit); above it, guided chunks over the resident blocks with spread
+// blocks_per_cu / (2 * resident), for launches whose blocks start unevenly
+// (CUs shared with the short and medium tasks' kernels).
+struct ChunkShape {
+  int64_t slots, spread, gran, min_chunk;
+  bool guided;
+};
+
+static ChunkShape chunk_shape(const DevFacts& f, const LaunchOpts& o) {
+  ChunkShape c;
+  const int res_cu = std::max(1, f.hist_blocks_per_cu);
+  c.guided = o.blocks_per_cu > res_cu;
+  c.slots = (int64_t)f.cus * (c.guided ? res_cu : std::max(1, o.blocks_per_cu));
+  c.spread = c.guided ? std::max(1, o.blocks_per_cu / (2 * res_cu)) : 1;
+  c.gran = (int64_t)BLOCK * 4 * o.unr;
+  c.min_chunk = (c.guided ? 8 : 4) * c.gran;
+  return c;
+}
+
+// k_split blocks of a task: one per 2^20 elements, up to 256, for tasks
+// longer than BIG_LEN.
+static inline int64_t split_blocks(int64_t len) { return len > BIG_LEN ? std::min<int64_t>(256, len >> 20) : 0; }
+
 // Long tasks are laid out predicted first, then the ones pass B always
 // re-reads, so pass B can cover just the latter's elements densely.
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
                 hipStream_t s) {
   const uint32_t scap = eff_small_cap(f, o);
-  int64_t nl = 0, ns = 0, nm = 0, tot = 0, tot_med = 0;
+  StageCounts cnt;
+  int64_t tot = 0, tot_med = 0, groups = 0;
   for (int64_t t = 0; t < n; ++t) {
     const int64_t len = tasks[t].len, k = tasks[t].k;
+    const int64_t g = o.predict ? sample_groups(len, k) : 0;
     if (is_short(len, scap)) {
-      ++ns;
-    } else if (is_medium(len, k, o.predict && sample_groups(len, k) > 0, o)) {
-      ++nm;
+      ++cnt.ns;
+    } else if (is_medium(len, k, g > 0, o)) {
+      ++cnt.nm;
       tot_med += len;
     } else {
-      ++nl;
+      ++cnt.nl;
       tot += len;
+      groups += g;
+      cnt.nsplit += split_blocks(len);
     }
   }
-  int rc = p.reserve(nl, ns, nm, tot + tot_med, s);
+  const ChunkShape cs = chunk_shape(f, o);
+  cnt.nch = cnt.nl ? plan_chunks(tot, cs.slots, cs.spread, cs.guided, cs.gran, cs.min_chunk, nullptr) : 0;
+  const int64_t grid_s = groups > 0 ? (groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK : 0;
+  cnt.nbmap = cnt.nch + grid_s;
+  int rc = p.reserve(cnt, tot + tot_med, s);
   if (rc) return rc;
-  int64_t il = 0, is = 0, im = 0, base = 0, mbase = tot, groups = 0, max_m = 0, runs = 0, msum = 0, tot_small = 0;
+  const int64_t nl = cnt.nl, ns = cnt.ns, nm = cnt.nm, nch = cnt.nch;
+  int64_t il = 0, is = 0, im = 0, base = 0, mbase = tot, max_m = 0, runs = 0, msum = 0, tot_small = 0;
   int64_t max_m_med = 0;
+  groups = 0;
   bool big = false, pos_bounds = false;
   // pass 0: predicted long tasks; pass 1: the others
   for (int pass = 0; pass < 2; ++pass)
@@ -384,38 +435,10 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.max_m_med = (uint32_t)max_m_med;
   // Launch shapes of the streaming kernels, pass A's chunk table and the
   // chunk -> task maps; each long task learns how many chunks share it.
-  // Pass A's table: with blocks_per_cu up to the resident pass-A blocks per
-  // CU, equal chunks for blocks_per_cu blocks per CU (one round when they
-  // all fit); above it, guided chunks over the resident blocks with spread
-  // blocks_per_cu / (2 * resident), for launches whose blocks start unevenly
-  // (CUs shared with the short and medium tasks' kernels).
-  p.grid_h = p.grid_s = 0;
-  const int res_cu = std::max(1, f.hist_blocks_per_cu);
-  const bool guided = o.blocks_per_cu > res_cu;
-  const int64_t slots = (int64_t)f.cus * (guided ? res_cu : std::max(1, o.blocks_per_cu));
-  const int64_t spread = guided ? std::max(1, o.blocks_per_cu / (2 * res_cu)) : 1;
-  const int64_t gran = (int64_t)BLOCK * 4 * o.unr;
-  const int64_t min_chunk = (guided ? 8 : 4) * gran;
-  const int64_t nch = nl ? plan_chunks(tot, slots, spread, guided, gran, min_chunk, nullptr) : 0;
-  if (nch > 0 && (!p.h_cstart || (size_t)nch + 1 > p.cap_cstart)) {
-    RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = 64;
-    while (c < (size_t)nch + 1) c *= 2;
-    if (p.h_cstart) RADIK_HIP(hipHostFree(p.h_cstart));
-    if (p.d_cstart) RADIK_HIP(hipFree(p.d_cstart));
-    p.h_cstart = nullptr;
-    p.d_cstart = nullptr;
-    if (hipHostMalloc((void**)&p.h_cstart, c * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&p.d_cstart, c * sizeof(int64_t)) != hipSuccess) {
-      p.cap_cstart = 0;
-      return set_error(6, "allocation of a %zu-entry chunk table failed", c);
-    }
-    p.cap_cstart = c;
-  }
-  if (nch > 0) plan_chunks(tot, slots, spread, guided, gran, min_chunk, p.h_cstart);
+  if (nch > 0) plan_chunks(tot, cs.slots, cs.spread, cs.guided, cs.gran, cs.min_chunk, p.h_cstart);
   p.n_chunks = nch;
   p.grid_h = (int)nch;
-  p.grid_s = groups > 0 ? (int)((groups + SAMPLE_GROUPS_PER_BLOCK - 1) / SAMPLE_GROUPS_PER_BLOCK) : 0;
+  p.grid_s = (int)grid_s;
   {
     int64_t c = 0;
     for (int64_t t = 0; t < nl; ++t) {
@@ -433,49 +456,14 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   }
   p.off_h = 0;
   p.off_s = p.off_h + nch;
-  p.n_bmap = p.off_s + p.grid_s;
-  if (!p.h_bmap || (size_t)p.n_bmap > p.cap_bmap) {
-    RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = 64;
-    while (c < (size_t)p.n_bmap) c *= 2;
-    if (p.h_bmap) RADIK_HIP(hipHostFree(p.h_bmap));
-    if (p.d_bmap) RADIK_HIP(hipFree(p.d_bmap));
-    p.h_bmap = nullptr;
-    p.d_bmap = nullptr;
-    if (hipHostMalloc((void**)&p.h_bmap, c * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&p.d_bmap, c * sizeof(uint32_t)) != hipSuccess) {
-      p.cap_bmap = 0;
-      return set_error(6, "allocation of a %zu-entry block map failed", c);
-    }
-    p.cap_bmap = c;
-  }
+  p.n_bmap = cnt.nbmap;
   fill_chunk_map(p.h_bmap + p.off_h, nch, p.h_cstart, p.h_bases, nl);
-  // k_split blocks of the tasks longer than BIG_LEN: one per 2^20 elements, up to 256
-  int64_t nsplit = 0;
-  for (int64_t t = 0; t < nl; ++t)
-    if (p.h_large[t].len > BIG_LEN) nsplit += std::min<int64_t>(256, p.h_large[t].len >> 20);
-  if (nsplit > 0 && (!p.h_xmap || (size_t)nsplit > p.cap_xmap)) {
-    RADIK_HIP(hipStreamSynchronize(s));
-    size_t c = 64;
-    while (c < (size_t)nsplit) c *= 2;
-    if (p.h_xmap) RADIK_HIP(hipHostFree(p.h_xmap));
-    if (p.d_xmap) RADIK_HIP(hipFree(p.d_xmap));
-    p.h_xmap = nullptr;
-    p.d_xmap = nullptr;
-    if (hipHostMalloc((void**)&p.h_xmap, c * sizeof(uint2), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&p.d_xmap, c * sizeof(uint2)) != hipSuccess) {
-      p.cap_xmap = 0;
-      return set_error(6, "allocation of a %zu-entry split map failed", c);
-    }
-    p.cap_xmap = c;
-  }
   int64_t ix = 0;
   for (int64_t t = 0; t < nl; ++t) {
-    if (p.h_large[t].len <= BIG_LEN) continue;
-    const uint32_t nb = (uint32_t)std::min<int64_t>(256, p.h_large[t].len >> 20);
+    const uint32_t nb = (uint32_t)split_blocks(p.h_large[t].len);
     for (uint32_t j = 0; j < nb; ++j) p.h_xmap[ix++] = make_uint2((uint32_t)t, j | (nb << 16));
   }
-  p.n_split = nsplit;
+  p.n_split = cnt.nsplit;
   fill_block_map(p.h_bmap + p.off_s, p.grid_s, 0, SAMPLE_GROUPS_PER_BLOCK, p.h_sbases, nl);
   p.total_runs = runs;
   p.total_m = msum;
@@ -490,24 +478,10 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
 }
 
 int upload(Pipeline& p, hipStream_t s) {
-  if (p.n_large) {
-    RADIK_HIP(hipMemcpyAsync(p.d_large, p.h_large, p.n_large * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
-    RADIK_HIP(hipMemcpyAsync(p.d_bases, p.h_bases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    RADIK_HIP(hipMemcpyAsync(p.d_sbases, p.h_sbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    RADIK_HIP(hipMemcpyAsync(p.d_cbases, p.h_cbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    RADIK_HIP(hipMemcpyAsync(p.d_mbases, p.h_mbases, (p.n_large + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    if (p.n_bmap)
-      RADIK_HIP(hipMemcpyAsync(p.d_bmap, p.h_bmap, p.n_bmap * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    if (p.n_chunks)
-      RADIK_HIP(hipMemcpyAsync(p.d_cstart, p.h_cstart, (p.n_chunks + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    if (p.n_split)
-      RADIK_HIP(hipMemcpyAsync(p.d_xmap, p.h_xmap, p.n_split * sizeof(uint2), hipMemcpyHostToDevice, s));
-  }
-  if (p.n_small)
-    RADIK_HIP(hipMemcpyAsync(p.d_small, p.h_small, p.n_small * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
-  if (p.n_med)
-    RADIK_HIP(hipMemcpyAsync(p.d_med, p.h_med, p.n_med * sizeof(TaskDesc), hipMemcpyHostToDevice, s));
-  RADIK_HIP(hipEventRecord(p.upload_done, s));
+  // one copy of every host-built array (the arena's prefix)
+  if (p.upload_bytes)
+    RADIK_HIP(hipMemcpyAsync(p.d_arena, p.h_arena, p.upload_bytes, hipMemcpyHostToDevice, s));
+  if (p.upload_done) RADIK_HIP(hipEventRecord(p.upload_done, s));
   return 0;
 }
 

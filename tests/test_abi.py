@@ -76,3 +76,4 @@ def test_bad_arguments_rejected_before_any_gpu_call():
     h = ctypes.c_void_p()
     assert L.radik_plan_create(ctypes.byref(h), lens, lens, 1, 0, 4, None) == _lib.RADIK_EINVAL
     assert L.radik_server_create(ctypes.byref(h), 0, 1, 0) == _lib.RADIK_EINVAL
+
