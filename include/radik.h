@@ -37,6 +37,7 @@ extern "C" {
 #define RADIK_ENOMEM        6 /* device or pinned allocation failed                  */
 #define RADIK_EADMIT        7 /* serving: request exceeds the workspace budget       */
 #define RADIK_ENOTREADY     8 /* serving: result not complete yet                    */
+#define RADIK_ELIMIT        9 /* radik_topk_noalloc: input beyond its fixed storage  */
 
 typedef struct radik_ws radik_ws;
 typedef struct radik_plan radik_plan;
@@ -74,6 +75,25 @@ int radik_topk_radix_select(radik_ws *ws, float *val_out, void *idx_out, int idx
 int radik_topk(float *val_out, void *idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
                const float *val_in, const void *indices, int64_t n_indices, const int64_t *task_lens,
                int64_t num_tasks, int64_t k, int largest, int rev, void *stream);
+
+/* Allocation-free variant of radik_topk.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, hosts that forbid hipMalloc).  Same
+ * arguments, validation and results as radik_topk, but the device work
+ * space is a fixed array inside the library (one copy per GPU, 50 MiB, held
+ * until the process ends) and the call allocates nothing on the device or
+ * the host.  Limits (RADIK_ELIMIT beyond them): num_tasks <=
+ * RADIK_NOALLOC_MAX_TASKS, sum(task_lens) <= RADIK_NOALLOC_MAX_ELEMS, at
+ * most RADIK_NOALLOC_MAX_LONG tasks longer than 2048 elements.  Calls are
+ * serialised inside the library and synchronise the stream before
+ * returning; every call on a GPU reuses that GPU's array (the library
+ * clears what must start at zero), so work on it must not overlap. */
+#define RADIK_NOALLOC_MAX_TASKS 8192
+#define RADIK_NOALLOC_MAX_ELEMS 4194304
+#define RADIK_NOALLOC_MAX_LONG 512
+int radik_topk_noalloc(float *val_out, void *idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
+                       const float *val_in, const void *indices, int64_t n_indices, const int64_t *task_lens,
+                       int64_t num_tasks, int64_t k, int largest, int rev, void *stream);
 
 /* Persistent batched form (no reference counterpart: the reference launches
  * one grid per pass, algorithm.jl:221-334).  A plan fixes task lengths, a k

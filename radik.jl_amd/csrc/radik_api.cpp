@@ -19,6 +19,8 @@ const char* last_error();
 int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStream_t s);
 int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s);
 int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
+int noalloc_run(float* val_out, void* idx_out, int idx_bytes, const float* val_in, const void* idx_in,
+                const int64_t* task_lens, int64_t num_tasks, int64_t k, uint32_t tflags, hipStream_t s);
 }  // namespace radik
 
 using namespace radik;
@@ -264,6 +266,43 @@ int radik_topk(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, i
   if (temp) radik_ws_destroy(ws);
   if (rc) return set_error(rc, "%s", keep.c_str());
   return 0;
+}
+
+int radik_topk_noalloc(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
+                       const float* val_in, const void* indices, int64_t n_indices, const int64_t* task_lens,
+                       int64_t num_tasks, int64_t k, int largest, int rev, void* stream) {
+  clear_error();
+  if (num_tasks < 0 || (num_tasks > 0 && !task_lens)) return set_error(RADIK_EINVAL, "bad task_lens");
+  if (out_rows != k || out_cols != num_tasks)
+    return set_error(RADIK_ESIZE, "val_out must have size (%lld, %lld), got (%lld, %lld)", (long long)k,
+                     (long long)num_tasks, (long long)out_rows, (long long)out_cols);
+  if (k < 0 || k > INT32_MAX) return set_error(RADIK_EINVAL, "k = %lld out of range", (long long)k);
+  if (idx_bytes != 4 && idx_bytes != 8) return set_error(RADIK_EINVAL, "idx_bytes must be 4 or 8");
+  int64_t total = 0, maxlen = 0;
+  int rc = check_lens(task_lens, num_tasks, idx_bytes, &total, &maxlen);
+  if (rc) return rc;
+  if (indices && n_indices > 0 && n_indices != total)
+    return set_error(RADIK_EINDICES, "indices length must equal data length. Got %lld vs %lld",
+                     (long long)n_indices, (long long)total);
+  if (num_tasks > RADIK_NOALLOC_MAX_TASKS)
+    return set_error(RADIK_ELIMIT, "radik_topk_noalloc takes at most %d tasks, got %lld", RADIK_NOALLOC_MAX_TASKS,
+                     (long long)num_tasks);
+  if (total > RADIK_NOALLOC_MAX_ELEMS)
+    return set_error(RADIK_ELIMIT, "radik_topk_noalloc takes at most %d elements in all, got %lld",
+                     RADIK_NOALLOC_MAX_ELEMS, (long long)total);
+  int64_t nlong = 0;
+  for (int64_t t = 0; t < num_tasks; ++t) nlong += task_lens[t] > 2048;
+  if (nlong > RADIK_NOALLOC_MAX_LONG)
+    return set_error(RADIK_ELIMIT, "radik_topk_noalloc takes at most %d tasks longer than 2048 elements, got %lld",
+                     RADIK_NOALLOC_MAX_LONG, (long long)nlong);
+  if (k == 0 || num_tasks == 0) return 0;
+  if (!val_out || !idx_out) return set_error(RADIK_EINVAL, "val_out/idx_out is NULL");
+  if (total > 0 && !val_in) return set_error(RADIK_EINVAL, "val_in is NULL");
+  const bool withidx = indices && n_indices > 0;
+  const uint32_t tflags = RADIK_WITHPACKING | (largest ? RADIK_LARGEST : 0u) | (rev ? 0u : RADIK_ASCEND) |
+                          (withidx ? RADIK_WITHIDXIN : 0u);
+  return noalloc_run(val_out, idx_out, idx_bytes, val_in, withidx ? indices : nullptr, task_lens, num_tasks, k,
+                     (tflags & 0x0Fu) | (idx_bytes == 8 ? F_IDX64 : 0u), (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------

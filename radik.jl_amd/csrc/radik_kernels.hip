@@ -2002,6 +2002,15 @@ __global__ __launch_bounds__(BLOCK) void k_diag_scan(const uint32_t* __restrict_
   }
 }
 
+// Zero fill of two ranges of 32-bit words (the histograms of a pipeline that
+// lives in fixed storage, cleared at the start of each of its launches).
+__global__ __launch_bounds__(BLOCK) void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restrict__ b,
+                                                 int64_t nb) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < na; i += stride) a[i] = 0u;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nb; i += stride) b[i] = 0u;
+}
+
 // Plain streaming copy (16-byte loads and stores), the reference point for
 // HBM traffic counters: it reads and writes exactly `n16` * 16 bytes.
 __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst,

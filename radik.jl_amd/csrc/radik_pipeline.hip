@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <mutex>
 
+#include "../../include/radik.h"
 #include "radik_host.h"
 #include "radik_kernels.hip"
 
@@ -691,6 +692,92 @@ int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
   const int grid = (int)std::min<int64_t>(want, (n16 + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL(k_diag_copy, dim3(grid), dim3(BLOCK), 0, s, (const f32x4*)src, (f32x4*)dst, n16);
   RADIK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free launches (radik_topk_noalloc): the pipeline lives in
+// `scratch`, a file-scope device array (one copy per GPU, held until the
+// process ends), and a static host arena.  Limits: NOALLOC_MAX_LONG long
+// tasks (histograms), NOALLOC_MAX_ELEMS elements of long and medium tasks
+// (spill regions), NOALLOC_MAX_TASKS tasks, NOALLOC_ARENA bytes of
+// descriptors.  The device address of `scratch` is taken once per device
+// with hipGetSymbolAddress (the name itself is not a device address in host
+// code).  Calls are serialised by one mutex and synchronise their stream
+// before returning; work another caller enqueues on the same GPU's scratch
+// outside this function (none exists in this library) would race.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_LONG = RADIK_NOALLOC_MAX_LONG;
+constexpr int64_t NOALLOC_MAX_ELEMS = RADIK_NOALLOC_MAX_ELEMS;
+constexpr int64_t NOALLOC_MAX_TASKS = RADIK_NOALLOC_MAX_TASKS;
+constexpr size_t NOALLOC_ARENA = (size_t)2 << 20;
+constexpr size_t NOALLOC_HIST = (size_t)NOALLOC_MAX_LONG * HIST_BINS * sizeof(uint32_t);
+constexpr size_t NOALLOC_BYTES = 2 * NOALLOC_HIST + (size_t)NOALLOC_MAX_ELEMS * sizeof(uint64_t) + NOALLOC_ARENA;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int kNoallocDevices = 64;
+std::mutex g_noalloc_mu;
+char* g_noalloc_dev[kNoallocDevices];  // device address of `scratch`, per device
+Pipeline g_noalloc_pipe[kNoallocDevices];
+alignas(256) char g_noalloc_host[NOALLOC_ARENA];
+TaskSpec g_noalloc_specs[NOALLOC_MAX_TASKS];
+}  // namespace
+
+// Runs the drop-in's pipeline for tasks described like radik_topk's
+// arguments (task t at the sum of earlier lengths, outputs [k, T]) in the
+// fixed storage; synchronises the stream.
+int noalloc_run(float* val_out, void* idx_out, int idx_bytes, const float* val_in, const void* idx_in,
+                const int64_t* task_lens, int64_t num_tasks, int64_t k, uint32_t tflags, hipStream_t s) {
+  if (num_tasks > NOALLOC_MAX_TASKS)
+    return set_error(RADIK_ELIMIT, "%lld tasks exceed the allocation-free limit of %lld", (long long)num_tasks,
+                     (long long)NOALLOC_MAX_TASKS);
+  int dev = 0;
+  RADIK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoallocDevices) return set_error(1, "device ordinal %d out of range", dev);
+  DevFacts* f = nullptr;
+  int rc = dev_facts(&f);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  Pipeline& p = g_noalloc_pipe[dev];
+  if (!g_noalloc_dev[dev]) {
+    void* ptr = nullptr;
+    RADIK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_error(5, "hipGetSymbolAddress returned no address for the scratch array");
+    rc = p.bind_fixed((char*)ptr, NOALLOC_BYTES, g_noalloc_host, NOALLOC_ARENA, NOALLOC_MAX_LONG,
+                      NOALLOC_MAX_ELEMS);
+    if (rc) return rc;
+    g_noalloc_dev[dev] = (char*)ptr;
+  }
+  int64_t off = 0;
+  for (int64_t t = 0; t < num_tasks; ++t) {
+    TaskSpec& sp = g_noalloc_specs[t];
+    sp.in = val_in + off;
+    sp.idx_in = (tflags & F_WITHIDXIN) ? (const void*)((const char*)idx_in + off * idx_bytes) : nullptr;
+    sp.val_out = val_out + t * k;
+    sp.idx_out = (void*)((char*)idx_out + t * k * idx_bytes);
+    sp.len = task_lens[t];
+    sp.k = k;
+    sp.flags = tflags;
+    off += task_lens[t];
+  }
+  LaunchOpts o;
+  rc = stage_tasks(p, g_noalloc_specs, num_tasks, *f, o, s);
+  if (rc) return rc;
+  // histograms start at zero (a fill kernel: no memset node in a stream
+  // that a caller may capture)
+  if (p.n_large > 0) {
+    const int64_t words = p.n_large * HIST_BINS;
+    const int grid = (int)std::min<int64_t>((words + BLOCK - 1) / BLOCK, 2048);
+    hipLaunchKernelGGL(k_zero2, dim3(grid), dim3(BLOCK), 0, s, p.d_hist, words, p.d_shist, words);
+    RADIK_HIP(hipGetLastError());
+  }
+  rc = upload(p, s);
+  if (rc) return rc;
+  rc = enqueue(p, *f, o, s, nullptr);
+  if (rc) return rc;
+  RADIK_HIP(hipStreamSynchronize(s));
   return 0;
 }
 

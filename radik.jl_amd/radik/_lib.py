@@ -24,6 +24,7 @@ RADIK_EHIP = 5
 RADIK_ENOMEM = 6
 RADIK_EADMIT = 7
 RADIK_ENOTREADY = 8
+RADIK_ELIMIT = 9
 
 
 class RadiKError(RuntimeError):
@@ -51,6 +52,7 @@ SIGNATURES = {
     "radik_ws_destroy": (_INT, [_P]),
     "radik_topk_radix_select": (_INT, [_P, _P, _P, _INT, _P, _P, _I64P, _I64, _I64, _U32, _INT, _P]),
     "radik_topk": (_INT, [_P, _P, _INT, _I64, _I64, _P, _P, _I64, _I64P, _I64, _I64, _INT, _INT, _P]),
+    "radik_topk_noalloc": (_INT, [_P, _P, _INT, _I64, _I64, _P, _P, _I64, _I64P, _I64, _I64, _INT, _INT, _P]),
     "radik_plan_create": (_INT, [ctypes.POINTER(_P), _I64P, _I64P, _I64, _U32, _INT, _P]),
     "radik_plan_execute": (_INT, [_P, _P, _P, _P, _P, _P]),
     "radik_plan_set_graph": (_INT, [_P, _INT]),

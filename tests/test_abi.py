@@ -77,3 +77,19 @@ def test_bad_arguments_rejected_before_any_gpu_call():
     assert L.radik_plan_create(ctypes.byref(h), lens, lens, 1, 0, 4, None) == _lib.RADIK_EINVAL
     assert L.radik_server_create(ctypes.byref(h), 0, 1, 0) == _lib.RADIK_EINVAL
 
+
+def test_noalloc_limits_rejected_before_any_gpu_call():
+    # the allocation-free entry point validates like radik_topk and rejects
+    # inputs beyond its fixed storage with RADIK_ELIMIT (host-side checks)
+    L = _lib.load()
+    n = (1 << 22) + 1
+    rc = L.radik_topk_noalloc(None, None, 4, 8, 1, None, None, 0, _lib.i64_array([n]), 1, 8, 1, 0, None)
+    assert rc == _lib.RADIK_ELIMIT and b"at most 4194304 elements" in L.radik_last_error()
+    rc = L.radik_topk_noalloc(None, None, 4, 8, 513, None, None, 0, _lib.i64_array([2049] * 513), 513, 8, 1, 0,
+                              None)
+    assert rc == _lib.RADIK_ELIMIT and b"longer than 2048" in L.radik_last_error()
+    rc = L.radik_topk_noalloc(None, None, 4, 8, 8193, None, None, 0, _lib.i64_array([1] * 8193), 8193, 8, 1, 0,
+                              None)
+    assert rc == _lib.RADIK_ELIMIT
+    rc = L.radik_topk_noalloc(None, None, 4, 50, 1, None, None, 0, _lib.i64_array([1000]), 1, 100, 1, 0, None)
+    assert rc == _lib.RADIK_ESIZE
