@@ -27,6 +27,20 @@ extern "C" {
 #define RADIK_WITHIDXIN   0x08u /* Val{WITHIDXIN}: idx_out = idx_in[pos]            */
 #define RADIK_WITHPACKING 0x10u /* Val{WITHPACKING}: vector loads (vload_bytes)    */
 
+/* Key type, bits 8-9 of the same flags word (no counterpart in the
+ * reference, which takes Float32 only: README.md:18-24,
+ * algorithm.jl:492-494).  The element arrays (val_in, val_out) of a call
+ * then hold that type; pass them cast to the declared pointer type.
+ * Ordering: the float types by their IEEE-754 total order (-NaN < -Inf <
+ * ... < -0 < +0 < ... < +Inf < +NaN), Int32 by value.  WITHSCALE takes
+ * Float32 keys only. */
+#define RADIK_KEY_SHIFT 8
+#define RADIK_KEY_F32  0x000u /* Float32 (default)              */
+#define RADIK_KEY_F16  0x100u /* Float16 (IEEE binary16)         */
+#define RADIK_KEY_BF16 0x200u /* BFloat16                        */
+#define RADIK_KEY_I32  0x300u /* Int32                           */
+#define RADIK_KEY_MASK 0x300u
+
 /* Status codes. */
 #define RADIK_OK            0
 #define RADIK_EINVAL        1 /* bad argument (k < 0, null pointer, bad flags)       */
@@ -76,6 +90,14 @@ int radik_topk(float *val_out, void *idx_out, int idx_bytes, int64_t out_rows, i
                const float *val_in, const void *indices, int64_t n_indices, const int64_t *task_lens,
                int64_t num_tasks, int64_t k, int largest, int rev, void *stream);
 
+/* topk! for any key type: key_type 0 Float32, 1 Float16, 2 BFloat16,
+ * 3 Int32 (RADIK_KEY_* >> RADIK_KEY_SHIFT); val_in / val_out hold that
+ * type.  Otherwise radik_topk (src/algorithm.jl:477-522 generalised past
+ * the reference's Float32-only check, algorithm.jl:492-494). */
+int radik_topk_typed(int key_type, void *val_out, void *idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
+                     const void *val_in, const void *indices, int64_t n_indices, const int64_t *task_lens,
+                     int64_t num_tasks, int64_t k, int largest, int rev, void *stream);
+
 /* Allocation-free variant of radik_topk.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under stream
  * capture, tight serving loops, hosts that forbid hipMalloc).  Same
@@ -103,7 +125,7 @@ int radik_topk_noalloc(float *val_out, void *idx_out, int idx_bytes, int64_t out
  * at the sum of earlier k's (equal k gives the [k, T] layout). */
 int radik_plan_create(radik_plan **plan, const int64_t *task_lens, const int64_t *k_per_task,
                       int64_t num_tasks, uint32_t flags, int idx_bytes, void *stream);
-int radik_plan_execute(radik_plan *plan, float *val_out, void *idx_out, const float *val_in,
+int radik_plan_execute(radik_plan *plan, void *val_out, void *idx_out, const void *val_in,
                        const void *idx_in, void *stream);
 /* use_graph = 0 launches the kernels eagerly (same results). */
 int radik_plan_set_graph(radik_plan *plan, int use_graph);
@@ -142,8 +164,8 @@ int radik_plan_destroy(radik_plan *plan);
  * the caller's val_out/idx_out (rows x k, row-major, int32 indices,
  * 1-based column positions). */
 int radik_server_create(radik_server **srv, int64_t budget_bytes, int max_inflight, uint32_t flags);
-int radik_server_submit(radik_server *srv, const float *in, int64_t rows, int64_t cols, int64_t k,
-                        float *val_out, int32_t *idx_out, int64_t *ticket);
+int radik_server_submit(radik_server *srv, const void *in, int64_t rows, int64_t cols, int64_t k,
+                        void *val_out, int32_t *idx_out, int64_t *ticket);
 /* Returns RADIK_OK when the ticket's results are complete (latency_ms: host
  * submit -> completion event), RADIK_ENOTREADY otherwise. */
 int radik_server_poll(radik_server *srv, int64_t ticket, double *latency_ms);

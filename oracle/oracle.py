@@ -75,6 +75,12 @@ def lib():
                                         ctypes.c_int64, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                         i64p, ctypes.c_int]
         L.orc_topk_schedule.restype = ctypes.c_int
+        L.orc_key_typed.argtypes = [ctypes.c_uint32, ctypes.c_int]
+        L.orc_key_typed.restype = ctypes.c_uint32
+        L.orc_topk_exact_typed.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, i64p,
+                                           ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_int]
+        L.orc_topk_exact_typed.restype = ctypes.c_int
         L.orc_nstat.restype = ctypes.c_int
         L.orc_max_threads.restype = ctypes.c_int
         _lib = L
@@ -205,3 +211,51 @@ def reorder(v, descending):
     output in one order (ties by position) into the other order."""
     kk = keys(v).astype(np.int64)
     return np.lexsort((np.arange(v.size), -kk if descending else kk))
+
+
+# Key types beyond Float32: numpy dtype of the element bit patterns per type.
+KEY_F32, KEY_F16, KEY_BF16, KEY_I32 = 0, 1, 2, 3
+KEY_DTYPES = {KEY_F32: np.float32, KEY_F16: np.float16, KEY_BF16: np.uint16, KEY_I32: np.int32}
+
+
+def key_typed(bits, kt):
+    return lib().orc_key_typed(int(bits), int(kt))
+
+
+def topk_exact_typed(kt, val_in, task_lens, k, flags, idx_in=None, idx_dtype=np.int32, threads=0):
+    """[T, k] values (the key type's bit patterns in KEY_DTYPES[kt]) and
+    indices, by the contract's direct order over the type's key.  BFloat16
+    data is passed as uint16 bit patterns."""
+    v = np.ascontiguousarray(val_in, dtype=KEY_DTYPES[kt])
+    tl, tlp = _i64(task_lens)
+    T = tl.size
+    idx_bytes = np.dtype(idx_dtype).itemsize
+    if idx_in is not None:
+        ii = np.ascontiguousarray(idx_in, dtype=idx_dtype)
+        flags |= WITHIDXIN
+        iptr = ii.ctypes.data
+    else:
+        flags &= ~WITHIDXIN
+        iptr = None
+    vo = np.zeros((T, k), dtype=KEY_DTYPES[kt])
+    io = np.zeros((T, k), dtype=idx_dtype)
+    rc = lib().orc_topk_exact_typed(kt, v.ctypes.data, iptr, idx_bytes, tlp, T, k, flags, vo.ctypes.data,
+                                    io.ctypes.data, threads)
+    if rc != 0:
+        raise ValueError("oracle rejected the arguments")
+    return vo, io
+
+
+def bf16_to_f32(bits):
+    """BFloat16 bit patterns (uint16) as the float32 values they denote."""
+    return (np.asarray(bits, np.uint16).astype(np.uint32) << np.uint32(16)).view(np.float32)
+
+
+def f32_to_bf16(x):
+    """Float32 values rounded to BFloat16 (nearest, ties to even) bit patterns."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + np.uint64(0x7FFF) + ((u >> np.uint64(16)) & np.uint64(1))) >> np.uint64(16)
+    out = r.astype(np.uint16)
+    nan = np.isnan(np.asarray(x, np.float32))
+    out[nan] = ((u[nan] >> np.uint64(16)) | np.uint64(0x40)).astype(np.uint16)
+    return out

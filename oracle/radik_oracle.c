@@ -427,6 +427,94 @@ int orc_topk_exact(const float *val_in, const void *idx_in, int idx_bytes, const
 }
 
 /* ------------------------------------------------------------------------ */
+/* Other key types (beyond the reference, which takes Float32 only:          */
+/* README.md:18-24, algorithm.jl:492-494): the same contract over the       */
+/* order-preserving key of each type.                                        */
+/*   kt 0 Float32  : orc_key (utils.jl:29-33)                                */
+/*   kt 1 Float16, 2 BFloat16 (16-bit patterns h):                           */
+/*        key16 = h ^ (sign ? 0xFFFF : 0x8000), the same closed form on 16   */
+/*        bits; IEEE-754 totalOrder of the values                            */
+/*   kt 3 Int32    : bits ^ 0x80000000 (two's complement order)              */
+/* Values are copied as bit patterns; WITHSCALE is Float32-only.             */
+/* ------------------------------------------------------------------------ */
+
+uint32_t orc_key_typed(uint32_t bits, int kt)
+{
+    if (kt == 0) return orc_key(u2f(bits));
+    if (kt == 3) return bits ^ 0x80000000u;
+    {
+        uint32_t h = bits & 0xFFFFu;
+        return h ^ ((h & 0x8000u) ? 0xFFFFu : 0x8000u);
+    }
+}
+
+static inline uint32_t load_bits(const void *p, int64_t i, int eb)
+{
+    if (eb == 2) return ((const uint16_t *)p)[i];
+    return ((const uint32_t *)p)[i];
+}
+
+static void topk_exact_task_typed(int kt, const void *data, int64_t n, int64_t k, uint32_t flags,
+                                  const void *idx_in, int idx_bytes, void *val_out, void *idx_out)
+{
+    const int eb = (kt == 1 || kt == 2) ? 2 : 4;
+    const int64_t m = n < k ? n : k;
+    const uint32_t xm = (flags & ORC_LARGEST) ? 0xFFFFFFFFu : 0u;
+    const uint32_t am = (flags & ORC_ASCEND) ? 0u : 0xFFFFFFFFu;
+    uint64_t *c = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++)
+        c[i] = ((uint64_t)(orc_key_typed(load_bits(data, i, eb), kt) ^ xm) << 32) | (uint64_t)i;
+    if (m > 0 && m < n) select_k_smallest(c, n, m);
+    /* output order: (ASCEND ? key : ~key, position) */
+    for (int64_t i = 0; i < m; i++) {
+        const int64_t p = (int64_t)(uint32_t)c[i];
+        c[i] = ((uint64_t)(orc_key_typed(load_bits(data, p, eb), kt) ^ am) << 32) | (uint64_t)p;
+    }
+    qsort(c, (size_t)m, sizeof(uint64_t), cmp_u64);
+    for (int64_t i = 0; i < k; i++) {
+        uint32_t v = 0;
+        int64_t id = -1;
+        if (i < m) {
+            const int64_t p = (int64_t)(uint32_t)c[i];
+            v = load_bits(data, p, eb);
+            if (flags & ORC_WITHIDXIN)
+                id = (idx_bytes == 8) ? ((const int64_t *)idx_in)[p] : (int64_t)((const int32_t *)idx_in)[p];
+            else
+                id = p + 1;
+        }
+        if (eb == 2) ((uint16_t *)val_out)[i] = (uint16_t)v;
+        else ((uint32_t *)val_out)[i] = v;
+        if (idx_bytes == 8) ((int64_t *)idx_out)[i] = id;
+        else ((int32_t *)idx_out)[i] = (int32_t)id;
+    }
+    free(c);
+}
+
+int orc_topk_exact_typed(int kt, const void *val_in, const void *idx_in, int idx_bytes, const int64_t *task_lens,
+                         int64_t num_tasks, int64_t k, uint32_t flags, void *val_out, void *idx_out, int threads)
+{
+    if (kt < 0 || kt > 3 || k < 0 || num_tasks < 0 || (idx_bytes != 4 && idx_bytes != 8)) return 1;
+    if (kt != 0 && (flags & ORC_WITHSCALE)) return 1;
+    const int eb = (kt == 1 || kt == 2) ? 2 : 4;
+    int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(num_tasks + 1));
+    off[0] = 0;
+    for (int64_t t = 0; t < num_tasks; t++) off[t + 1] = off[t] + task_lens[t];
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t t = 0; t < num_tasks; t++) {
+        const void *ii = (flags & ORC_WITHIDXIN) ? (const char *)idx_in + (size_t)off[t] * idx_bytes : NULL;
+        topk_exact_task_typed(kt, (const char *)val_in + (size_t)off[t] * eb, task_lens[t], k, flags, ii,
+                              idx_bytes, (char *)val_out + (size_t)t * k * eb,
+                              (char *)idx_out + (size_t)t * k * idx_bytes);
+    }
+    (void)threads;
+    free(off);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* The reference's schedule                                                  */
 /* ------------------------------------------------------------------------ */
 

@@ -47,9 +47,9 @@ struct radik_plan {
   LaunchOpts opts;
   std::vector<TaskSpec> specs;
   // pointers the staged descriptors were built for
-  const float* last_in = nullptr;
+  const void* last_in = nullptr;
   const void* last_idx_in = nullptr;
-  float* last_val_out = nullptr;
+  void* last_val_out = nullptr;
   void* last_idx_out = nullptr;
   bool staged = false;
   // graph
@@ -77,10 +77,21 @@ struct TopkCache {
 };
 static TopkCache g_topk_cache[kMaxDevices];
 
-static const uint32_t kFlagMask = RADIK_LARGEST | RADIK_ASCEND | RADIK_WITHSCALE | RADIK_WITHIDXIN | RADIK_WITHPACKING;
+static const uint32_t kFlagMask =
+    RADIK_LARGEST | RADIK_ASCEND | RADIK_WITHSCALE | RADIK_WITHIDXIN | RADIK_WITHPACKING | RADIK_KEY_MASK;
+
+static inline int key_type_of(uint32_t flags) { return (int)((flags & RADIK_KEY_MASK) >> RADIK_KEY_SHIFT); }
+
+// WITHSCALE subtracts in Float32 (utils.jl:62-76); other key types reject it.
+static int check_key_flags(uint32_t flags) {
+  if ((flags & RADIK_WITHSCALE) && key_type_of(flags) != KEY_F32)
+    return set_error(RADIK_EINVAL, "WITHSCALE needs Float32 keys");
+  return 0;
+}
 
 static LaunchOpts opts_for(uint32_t flags, int vload_bytes) {
   LaunchOpts o;
+  o.key_type = key_type_of(flags);
   if (!(flags & RADIK_WITHPACKING)) o.unr = 1;
   else if (vload_bytes >= 64) o.unr = 4;
   else if (vload_bytes >= 32) o.unr = 2;
@@ -168,18 +179,19 @@ static int run_tasks(Pipeline& pipe, std::vector<TaskSpec>& specs, const LaunchO
   return enqueue(pipe, *f, o, s, tm);
 }
 
-int radik_topk_radix_select(radik_ws* ws, float* val_out, void* idx_out, int idx_bytes, const float* val_in,
-                            const void* idx_in, const int64_t* task_lens, int64_t num_tasks, int64_t k,
-                            uint32_t flags, int vload_bytes, void* stream) {
-  clear_error();
+static int ws_select(radik_ws* ws, void* val_out, void* idx_out, int idx_bytes, const void* val_in,
+                     const void* idx_in, const int64_t* task_lens, int64_t num_tasks, int64_t k, uint32_t flags,
+                     int vload_bytes, void* stream) {
   if (!ws) return set_error(RADIK_EINVAL, "workspace is NULL");
   if (k < 0 || k > INT32_MAX) return set_error(RADIK_EINVAL, "k = %lld out of range", (long long)k);
   if (num_tasks < 0) return set_error(RADIK_EINVAL, "num_tasks < 0");
   if (num_tasks > 0 && !task_lens) return set_error(RADIK_EINVAL, "task_lens is NULL");
   if (idx_bytes != 4 && idx_bytes != 8) return set_error(RADIK_EINVAL, "idx_bytes must be 4 or 8");
   if (flags & ~kFlagMask) return set_error(RADIK_EINVAL, "unknown flag bits 0x%x", flags & ~kFlagMask);
+  int rc = check_key_flags(flags);
+  if (rc) return rc;
   int64_t total = 0, maxlen = 0;
-  int rc = check_lens(task_lens, num_tasks, idx_bytes, &total, &maxlen);
+  rc = check_lens(task_lens, num_tasks, idx_bytes, &total, &maxlen);
   if (rc) return rc;
   if (maxlen > ws->n)
     return set_error(RADIK_EWORKSPACE, "workspace stride n = %lld < max(task_lens) = %lld", (long long)ws->n,
@@ -200,13 +212,14 @@ int radik_topk_radix_select(radik_ws* ws, float* val_out, void* idx_out, int idx
                                           ws->device, cur);
   hipStream_t s = (hipStream_t)stream;
   const uint32_t tflags = (flags & 0x0Fu) | (idx_bytes == 8 ? F_IDX64 : 0u);
+  const int eb = key_bytes(key_type_of(flags));
   ws->specs.resize((size_t)num_tasks);
   int64_t off = 0;
   for (int64_t t = 0; t < num_tasks; ++t) {
     TaskSpec& sp = ws->specs[(size_t)t];
-    sp.in = val_in + off;
+    sp.in = elem_ptr(val_in, off, eb);
     sp.idx_in = (flags & RADIK_WITHIDXIN) ? (const void*)((const char*)idx_in + off * idx_bytes) : nullptr;
-    sp.val_out = val_out + t * k;
+    sp.val_out = elem_ptr(val_out, t * k, eb);
     sp.idx_out = (void*)((char*)idx_out + t * k * idx_bytes);
     sp.len = task_lens[t];
     sp.k = k;
@@ -221,10 +234,18 @@ int radik_topk_radix_select(radik_ws* ws, float* val_out, void* idx_out, int idx
   return 0;
 }
 
-int radik_topk(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
-               const float* val_in, const void* indices, int64_t n_indices, const int64_t* task_lens,
-               int64_t num_tasks, int64_t k, int largest, int rev, void* stream) {
+int radik_topk_radix_select(radik_ws* ws, float* val_out, void* idx_out, int idx_bytes, const float* val_in,
+                            const void* idx_in, const int64_t* task_lens, int64_t num_tasks, int64_t k,
+                            uint32_t flags, int vload_bytes, void* stream) {
   clear_error();
+  return ws_select(ws, val_out, idx_out, idx_bytes, val_in, idx_in, task_lens, num_tasks, k, flags, vload_bytes,
+                   stream);
+}
+
+static int topk_impl(int key_type, void* val_out, void* idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
+                     const void* val_in, const void* indices, int64_t n_indices, const int64_t* task_lens,
+                     int64_t num_tasks, int64_t k, int largest, int rev, void* stream) {
+  if (key_type < 0 || key_type >= NUM_KEY_TYPES) return set_error(RADIK_EINVAL, "unknown key type %d", key_type);
   if (num_tasks < 0 || (num_tasks > 0 && !task_lens)) return set_error(RADIK_EINVAL, "bad task_lens");
   if (out_rows != k || out_cols != num_tasks)
     return set_error(RADIK_ESIZE, "val_out must have size (%lld, %lld), got (%lld, %lld)", (long long)k,
@@ -235,7 +256,8 @@ int radik_topk(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, i
   if (indices && n_indices > 0 && n_indices != total)
     return set_error(RADIK_EINDICES, "indices length must equal data length. Got %lld vs %lld",
                      (long long)n_indices, (long long)total);
-  uint32_t flags = RADIK_WITHPACKING | (largest ? RADIK_LARGEST : 0u) | (rev ? 0u : RADIK_ASCEND);
+  uint32_t flags = RADIK_WITHPACKING | (largest ? RADIK_LARGEST : 0u) | (rev ? 0u : RADIK_ASCEND) |
+                   ((uint32_t)key_type << RADIK_KEY_SHIFT);
   if (indices && n_indices > 0) flags |= RADIK_WITHIDXIN;
   // The reference allocates a workspace per call (algorithm.jl:503).  Here
   // each device keeps one cached workspace for these calls (grown as needed,
@@ -260,12 +282,27 @@ int radik_topk(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, i
     if (ws->n < maxlen) ws->n = maxlen;
     if (ws->num_tasks < num_tasks) ws->num_tasks = num_tasks;
   }
-  rc = radik_topk_radix_select(ws, val_out, idx_out, idx_bytes, val_in, indices, task_lens, num_tasks, k, flags,
-                               32, stream);
+  rc = ws_select(ws, val_out, idx_out, idx_bytes, val_in, indices, task_lens, num_tasks, k, flags, 32, stream);
   std::string keep = rc ? std::string(radik_last_error()) : std::string();
   if (temp) radik_ws_destroy(ws);
   if (rc) return set_error(rc, "%s", keep.c_str());
   return 0;
+}
+
+int radik_topk(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
+               const float* val_in, const void* indices, int64_t n_indices, const int64_t* task_lens,
+               int64_t num_tasks, int64_t k, int largest, int rev, void* stream) {
+  clear_error();
+  return topk_impl(KEY_F32, val_out, idx_out, idx_bytes, out_rows, out_cols, val_in, indices, n_indices, task_lens,
+                   num_tasks, k, largest, rev, stream);
+}
+
+int radik_topk_typed(int key_type, void* val_out, void* idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
+                     const void* val_in, const void* indices, int64_t n_indices, const int64_t* task_lens,
+                     int64_t num_tasks, int64_t k, int largest, int rev, void* stream) {
+  clear_error();
+  return topk_impl(key_type, val_out, idx_out, idx_bytes, out_rows, out_cols, val_in, indices, n_indices,
+                   task_lens, num_tasks, k, largest, rev, stream);
 }
 
 int radik_topk_noalloc(float* val_out, void* idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
@@ -317,8 +354,10 @@ int radik_plan_create(radik_plan** plan, const int64_t* task_lens, const int64_t
     return set_error(RADIK_EINVAL, "bad task_lens / k_per_task");
   if (idx_bytes != 4 && idx_bytes != 8) return set_error(RADIK_EINVAL, "idx_bytes must be 4 or 8");
   if (flags & ~kFlagMask) return set_error(RADIK_EINVAL, "unknown flag bits 0x%x", flags & ~kFlagMask);
+  int rc = check_key_flags(flags);
+  if (rc) return rc;
   int64_t total = 0, maxlen = 0;
-  int rc = check_lens(task_lens, num_tasks, idx_bytes, &total, &maxlen);
+  rc = check_lens(task_lens, num_tasks, idx_bytes, &total, &maxlen);
   if (rc) return rc;
   std::unique_ptr<radik_plan> p(new (std::nothrow) radik_plan());
   if (!p) return set_error(RADIK_ENOMEM, "out of host memory");
@@ -387,7 +426,7 @@ int radik_plan_timing(radik_plan* p, double* ms8, int64_t* count) {
   return 0;
 }
 
-int radik_plan_execute(radik_plan* p, float* val_out, void* idx_out, const float* val_in, const void* idx_in,
+int radik_plan_execute(radik_plan* p, void* val_out, void* idx_out, const void* val_in, const void* idx_in,
                        void* stream) {
   clear_error();
   if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
@@ -403,12 +442,13 @@ int radik_plan_execute(radik_plan* p, float* val_out, void* idx_out, const float
                     p->last_idx_out == idx_out;
   if (!same) {
     const uint32_t tflags = (p->flags & 0x0Fu) | (p->idx_bytes == 8 ? F_IDX64 : 0u);
+    const int eb = key_bytes(key_type_of(p->flags));
     for (int64_t t = 0; t < T; ++t) {
       TaskSpec& sp = p->specs[(size_t)t];
       const int64_t io = p->in_off[(size_t)t], oo = p->out_off[(size_t)t];
-      sp.in = val_in + io;
+      sp.in = elem_ptr(val_in, io, eb);
       sp.idx_in = (p->flags & RADIK_WITHIDXIN) ? (const void*)((const char*)idx_in + io * p->idx_bytes) : nullptr;
-      sp.val_out = val_out + oo;
+      sp.val_out = elem_ptr(val_out, oo, eb);
       sp.idx_out = (void*)((char*)idx_out + oo * p->idx_bytes);
       sp.len = p->lens[(size_t)t];
       sp.k = p->ks[(size_t)t];
@@ -503,6 +543,7 @@ int radik_diag_pass1(const float* val_in, const int64_t* task_lens, int64_t num_
                      uint32_t* hist_out, uint32_t* sel_out, void* stream) {
   clear_error();
   if (num_tasks <= 0 || !task_lens || !val_in || !hist_out || !sel_out) return set_error(RADIK_EINVAL, "bad arguments");
+  if (flags & RADIK_KEY_MASK) return set_error(RADIK_EINVAL, "radik_diag_pass1 takes Float32 keys");
   if (k < 0 || k > INT32_MAX) return set_error(RADIK_EINVAL, "bad k");
   int64_t total = 0, maxlen = 0;
   int rc = check_lens(task_lens, num_tasks, 8, &total, &maxlen);

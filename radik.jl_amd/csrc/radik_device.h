@@ -15,19 +15,36 @@ constexpr uint32_t F_WITHSCALE = 0x04u;
 constexpr uint32_t F_WITHIDXIN = 0x08u;
 constexpr uint32_t F_IDX64 = 0x100u;
 
+// Key types.  The kernels are compiled once per key type (RADIK_DT selects
+// it in a kernel translation unit); a launch holds tasks of one type.
+constexpr int KEY_F32 = 0;   // float32 (the reference's only type)
+constexpr int KEY_F16 = 1;   // IEEE binary16
+constexpr int KEY_BF16 = 2;  // bfloat16
+constexpr int KEY_I32 = 3;   // int32
+constexpr int NUM_KEY_TYPES = 4;
+inline int key_bytes(int kt) { return (kt == KEY_F16 || kt == KEY_BF16) ? 2 : 4; }
+
 constexpr uint32_t NO_BIN = 0xFFFFFFFFu;
 constexpr int HIST_BINS = 4096;  // first digit: rank-key bits [31:20]
 constexpr int BLOCK = 256;       // threads of every kernel in this library
 // Predictions are refined to a composite bound only for samples of at most
 // this many groups (the host picks pass A's variant by it).
 constexpr int PRED_MAX_G = 64;
+// Sampling: groups of SAMPLE_GROUP consecutive elements, SAMPLE_GROUPS_PER_BLOCK
+// per block of k_sample.
+constexpr int SAMPLE_GROUP = BLOCK;
+constexpr int SAMPLE_GROUPS_PER_BLOCK = 16;
+// Multi-block sort of large result sets: runs of SORT_CHUNK, at most
+// SORT_MAX_RUNS of them per task.
+constexpr uint32_t SORT_CHUNK = BLOCK * 8;
+constexpr uint32_t SORT_MAX_RUNS = 64;
 
 // One task of a launch.  Pointers address the task's own first element, so a
 // launch may mix tasks from different buffers (the serving engine does).
 struct TaskDesc {
-  const float* in;      // len values
+  const void* in;       // len values of the launch's key type
   const void* idx_in;   // len indices (int32/int64) when F_WITHIDXIN
-  float* val_out;       // k values
+  void* val_out;        // k values of the key type
   void* idx_out;        // k indices (int32/int64)
   int64_t len;          // elements (< 2^32)
   int64_t base;         // start in the launch's concatenated element space;
@@ -83,26 +100,6 @@ __device__ __forceinline__ uint32_t fkey(float x) {
 __device__ __forceinline__ float fkey_inv(uint32_t k) {
   uint32_t u = (k & 0x80000000u) ? (k ^ 0x80000000u) : ~k;
   return __uint_as_float(u);
-}
-
-// Scaled value: x - s for WITHSCALE tasks (s = 0 otherwise, which is exact
-// for every non-NaN x); NaN keeps its own bits.
-__device__ __forceinline__ float scaled(float x, float s) {
-  float d = x - s;
-  return (x == x) ? d : x;
-}
-
-// Rank key: smaller is better.  xm = 0xFFFFFFFF for LARGEST, 0 otherwise.
-__device__ __forceinline__ uint32_t rank_key(float x, float s, uint32_t xm) {
-  return fkey(scaled(x, s)) ^ xm;
-}
-
-// The WITHSCALE scaler of a task: its first element, 0 if NaN/Inf
-// (src/utils.jl:62-66); 0 when the task does not scale.
-__device__ __forceinline__ float task_scaler(const TaskDesc& d) {
-  if (!(d.flags & F_WITHSCALE) || d.len <= 0) return 0.0f;
-  float s = glb(d.in)[0];
-  return (isnan(s) || isinf(s)) ? 0.0f : s;
 }
 
 __device__ __forceinline__ uint32_t lane_id() {

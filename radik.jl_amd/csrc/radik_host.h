@@ -32,7 +32,7 @@ struct DevFacts {
   uint32_t refine_cap = 4096;  // default: candidates kept in LDS by k_finish
   uint32_t small_cap_max = 2048, refine_cap_max = 4096;  // what the LDS budget allows
   uint32_t sort_cap = 4096;    // results sorted in LDS by k_sort
-  int hist_blocks_per_cu = 4;  // resident pass-A blocks per CU (occupancy of k_hist)
+  int hist_blocks_per_cu[NUM_KEY_TYPES] = {4, 4, 4, 4};  // resident pass-A blocks per CU, per key type
 };
 int dev_facts(DevFacts** out);
 
@@ -56,6 +56,7 @@ struct LaunchOpts {
   uint32_t refine_cap = 0;  // candidates held in LDS by k_finish (0: device default)
   bool predict = true;      // sample long tasks and spill in pass A
   bool medium = true;       // medium tasks to k_medium (else the streaming passes)
+  int key_type = KEY_F32;   // element type of every task of the launch (KEY_*)
 };
 inline uint32_t eff_small_cap(const DevFacts& f, const LaunchOpts& o) {
   return o.small_cap ? (o.small_cap < f.small_cap_max ? o.small_cap : f.small_cap_max) : f.small_cap;
@@ -147,9 +148,9 @@ struct Pipeline {
 // Splits tasks into short ones (k_small) and long ones (streaming passes),
 // fills the pinned descriptor arrays and their prefix sums.
 struct TaskSpec {
-  const float* in;
+  const void* in;  // elements of the launch's key type
   const void* idx_in;
-  float* val_out;
+  void* val_out;
   void* idx_out;
   int64_t len;
   int64_t k;
@@ -157,6 +158,14 @@ struct TaskSpec {
 };
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
                 hipStream_t s);
+
+// Address of element i of an array of eb-byte elements.
+inline const void* elem_ptr(const void* base, int64_t i, int eb) {
+  return base ? (const void*)((const char*)base + i * (int64_t)eb) : nullptr;
+}
+inline void* elem_ptr(void* base, int64_t i, int eb) {
+  return base ? (void*)((char*)base + i * (int64_t)eb) : nullptr;
+}
 
 // Phase timing hooks (events recorded between kernels when non-null).
 // Phases: sample, predict, hist, select, compact, finish, sort, small.

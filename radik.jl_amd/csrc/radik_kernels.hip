@@ -42,10 +42,95 @@
 // element granularity; a chunk that crosses task boundaries flushes per task.
 // k_sample takes equal chunks; k_hist and k_compact take the chunks of a host
 // table, shrinking towards its end; one chunk per block.
+//
+// Key types: this file is compiled once per key type (RADIK_DT: float32,
+// float16, bfloat16, int32; radik_launch.hip), each copy in its own
+// namespace.  Elements enter as their bit patterns; the order-preserving
+// 32-bit key of a 16-bit type is its 16-bit key in the upper half (the lower
+// half is zero, so the digit passes below bit 16 fix themselves without a
+// pass).
 
 #include "radik_device.h"
 
+#ifndef RADIK_DT
+#define RADIK_DT 0
+#endif
+#ifndef RADIK_KNS
+#define RADIK_KNS kf32
+#endif
+
 namespace radik {
+namespace RADIK_KNS {
+
+constexpr int DT = RADIK_DT;
+constexpr int EB = (DT == KEY_F16 || DT == KEY_BF16) ? 2 : 4;  // element bytes
+constexpr int EPV = 16 / EB;                                   // elements per 16-byte load
+
+// Order-preserving 32-bit key of an element's bits (unsigned order of keys
+// == numeric order; -0 < +0 and NaNs outside +-Inf for the float types;
+// RadiK.jl src/utils.jl:29-33 for float32), and its inverse.
+__device__ __forceinline__ uint32_t ekey(uint32_t bits) {
+  if constexpr (DT == KEY_F32) {
+    return bits ^ ((uint32_t)((int32_t)bits >> 31) | 0x80000000u);
+  } else if constexpr (DT == KEY_I32) {
+    return bits ^ 0x80000000u;
+  } else {
+    const uint32_t h = bits & 0xFFFFu;
+    return (h ^ ((h & 0x8000u) ? 0xFFFFu : 0x8000u)) << 16;
+  }
+}
+
+__device__ __forceinline__ uint32_t ekey_inv(uint32_t k) {
+  if constexpr (DT == KEY_F32) {
+    return (k & 0x80000000u) ? (k ^ 0x80000000u) : ~k;
+  } else if constexpr (DT == KEY_I32) {
+    return k ^ 0x80000000u;
+  } else {
+    const uint32_t h = k >> 16;
+    return ((h & 0x8000u) ? (h ^ 0x8000u) : ~h) & 0xFFFFu;
+  }
+}
+
+// Element i of an array of the key type, as bits.
+__device__ __forceinline__ uint32_t load_elem(const void* p, int64_t i) {
+  if constexpr (EB == 2) return (uint32_t)glb(reinterpret_cast<const uint16_t*>(p))[i];
+  else return glb(reinterpret_cast<const uint32_t*>(p))[i];
+}
+
+__device__ __forceinline__ void store_elem(void* p, int64_t i, uint32_t bits) {
+  if constexpr (EB == 2) glb(reinterpret_cast<uint16_t*>(p))[i] = (uint16_t)bits;
+  else glb(reinterpret_cast<uint32_t*>(p))[i] = bits;
+}
+
+// Scaled value (WITHSCALE, float32 only): x - s; NaN keeps its own bits.
+__device__ __forceinline__ float scaled(float x, float s) {
+  float d = x - s;
+  return (x == x) ? d : x;
+}
+
+// Rank key of an element: smaller is better.  xm = 0xFFFFFFFF for LARGEST,
+// 0 otherwise; sc the task's scaler (WITHSCALE float32 tasks, else unused).
+template <bool SCALE>
+__device__ __forceinline__ uint32_t rank_bits(uint32_t bits, float sc, uint32_t xm) {
+  if constexpr (SCALE && DT == KEY_F32) return ekey(__float_as_uint(scaled(__uint_as_float(bits), sc))) ^ xm;
+  else return ekey(bits) ^ xm;
+}
+
+__device__ __forceinline__ uint32_t rank_key(uint32_t bits, float sc, uint32_t xm, bool scale) {
+  return scale ? rank_bits<true>(bits, sc, xm) : rank_bits<false>(bits, sc, xm);
+}
+
+// The WITHSCALE scaler of a task: its first element, 0 if NaN/Inf
+// (src/utils.jl:62-66); 0 when the task does not scale.
+__device__ __forceinline__ float task_scaler(const TaskDesc& d) {
+  if (DT != KEY_F32 || !(d.flags & F_WITHSCALE) || d.len <= 0) return 0.0f;
+  float s = __uint_as_float(load_elem(d.in, 0));
+  return (isnan(s) || isinf(s)) ? 0.0f : s;
+}
+
+__device__ __forceinline__ bool task_scales(const TaskDesc& d) {
+  return DT == KEY_F32 && (d.flags & F_WITHSCALE) != 0;
+}
 
 // Largest t with bases[t] <= g (bases: T+1 prefix sums, bases[T] = total).
 __device__ __forceinline__ int find_task(const int64_t* __restrict__ bases, int ntasks, int64_t g) {
@@ -58,67 +143,64 @@ __device__ __forceinline__ int find_task(const int64_t* __restrict__ bases, int 
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // The vector body of a streaming read starts on a 128-byte boundary (after a
-// scalar head of at most 31 elements), so each wave's 1 KiB load covers
+// scalar head of less than 128 bytes), so each wave's 1 KiB load covers
 // whole cache lines: with streaming (non-temporal) loads a line shared by
 // two wave loads is fetched from HBM twice (6 % more traffic on a batch of
 // arbitrarily aligned tasks).
-constexpr uint32_t BODY_ALIGN = 32;  // elements of 4 bytes
+constexpr uint32_t BODY_ALIGN = 128 / EB;  // elements
 
-__device__ __forceinline__ int64_t aligned_head_end(const float* p, int64_t a, int64_t e) {
-  const int64_t mis = (int64_t)(((uintptr_t)(p + a) >> 2) & (BODY_ALIGN - 1));
+__device__ __forceinline__ int64_t aligned_head_end(const void* p, int64_t a, int64_t e) {
+  const int64_t mis = (int64_t)((((uintptr_t)p / EB) + (uint64_t)a) & (BODY_ALIGN - 1));
   const int64_t h = a + (((int64_t)BODY_ALIGN - mis) & (int64_t)(BODY_ALIGN - 1));
   return h < e ? h : e;
 }
 
-// Visits elements [s, e) of task array p: scalar head up to 128-byte
-// alignment, UNR float4 loads in flight per thread over the body, scalar
-// tail.  f(x, pos).
+// Element j (0 <= j < EPV) of a 16-byte vector, as bits.
+__device__ __forceinline__ uint32_t vec_elem(const u32x4& r, int j) {
+  if constexpr (EB == 4) return r[j];
+  else return (r[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+}
+
+// Visits elements [s, e) of task array pp: scalar head up to 128-byte
+// alignment, UNR 16-byte loads in flight per thread over the body, scalar
+// tail.  f(bits, pos).
 template <int UNR, typename F>
-__device__ __forceinline__ void stream_range(const float* __restrict__ pp, int64_t s, int64_t e, F&& f) {
-  const RADIK_AS1 float* __restrict__ p = glb(pp);
+__device__ __forceinline__ void stream_range(const void* __restrict__ pp, int64_t s, int64_t e, F&& f) {
   const int64_t tid = threadIdx.x;
   int64_t a = s;
   const int64_t head_end = aligned_head_end(pp, a, e);
-  for (int64_t i = a + tid; i < head_end; i += BLOCK) f(p[i], (uint32_t)i);
+  for (int64_t i = a + tid; i < head_end; i += BLOCK) f(load_elem(pp, i), (uint32_t)i);
   a = head_end;
-  const int64_t nvec = (e - a) >> 2;
-  const RADIK_AS1 f32x4* __restrict__ pv = reinterpret_cast<const RADIK_AS1 f32x4*>(p + a);
+  const int64_t nvec = (e - a) / EPV;
+  const RADIK_AS1 u32x4* __restrict__ pv =
+      reinterpret_cast<const RADIK_AS1 u32x4*>(glb(reinterpret_cast<const char*>(pp)) + a * EB);
   int64_t v = tid;
   for (; v + (int64_t)(UNR - 1) * BLOCK < nvec; v += (int64_t)UNR * BLOCK) {
-    f32x4 r[UNR];
+    u32x4 r[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) r[u] = __builtin_nontemporal_load(pv + v + (int64_t)u * BLOCK);
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      const uint32_t q = (uint32_t)(a + ((v + (int64_t)u * BLOCK) << 2));
-      f(r[u].x, q);
-      f(r[u].y, q + 1);
-      f(r[u].z, q + 2);
-      f(r[u].w, q + 3);
+      const uint32_t q = (uint32_t)(a + (v + (int64_t)u * BLOCK) * EPV);
+#pragma unroll
+      for (int j = 0; j < EPV; ++j) f(vec_elem(r[u], j), q + (uint32_t)j);
     }
   }
   for (; v < nvec; v += BLOCK) {
-    f32x4 r = __builtin_nontemporal_load(pv + v);
-    const uint32_t q = (uint32_t)(a + (v << 2));
-    f(r.x, q);
-    f(r.y, q + 1);
-    f(r.z, q + 2);
-    f(r.w, q + 3);
+    const u32x4 r = __builtin_nontemporal_load(pv + v);
+    const uint32_t q = (uint32_t)(a + v * EPV);
+#pragma unroll
+    for (int j = 0; j < EPV; ++j) f(vec_elem(r, j), q + (uint32_t)j);
   }
-  for (int64_t i = a + (nvec << 2) + tid; i < e; i += BLOCK) f(p[i], (uint32_t)i);
+  for (int64_t i = a + nvec * EPV + tid; i < e; i += BLOCK) f(load_elem(pp, i), (uint32_t)i);
 }
 
-// Rank key with the task's scaling decided at compile time.
-template <bool SCALE>
-__device__ __forceinline__ uint32_t rk_of(float x, float sc, uint32_t xm) {
-  return SCALE ? (fkey(scaled(x, sc)) ^ xm) : (fkey(x) ^ xm);
-}
-
-// Stores a result (value, position) into output slot `slot` of task d.
-__device__ __forceinline__ void put_result(const TaskDesc& d, uint32_t slot, float v, uint32_t pos) {
-  glb(d.val_out)[slot] = v;
+// Stores a result (value bits, position) into output slot `slot` of task d.
+__device__ __forceinline__ void put_result(const TaskDesc& d, uint32_t slot, uint32_t bits, uint32_t pos) {
+  store_elem(d.val_out, slot, bits);
   if (d.flags & F_IDX64) glb(reinterpret_cast<int64_t*>(d.idx_out))[slot] = (int64_t)pos;
   else glb(reinterpret_cast<int32_t*>(d.idx_out))[slot] = (int32_t)pos;
 }
@@ -334,8 +416,6 @@ __device__ __forceinline__ uint64_t sample_upper(const uint64_t* v, uint32_t n, 
 // equal runs of groups of the launch; per task the first-digit histogram of
 // the sample goes to shist.
 // ---------------------------------------------------------------------------
-constexpr int SAMPLE_GROUP = BLOCK;
-constexpr int SAMPLE_GROUPS_PER_BLOCK = 16;
 
 __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ tasks,
                                                   const int64_t* __restrict__ sbases, int ntasks,
@@ -354,20 +434,21 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
     const int64_t G = sbases[t + 1] - tb;
     const int64_t se = min(g1, tb + G);
     const float sc = task_scaler(d);
+    const bool scale = task_scales(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
     // all of this task's groups of the block in flight at once
-    float xv[SAMPLE_GROUPS_PER_BLOCK];
+    uint32_t xv[SAMPLE_GROUPS_PER_BLOCK];
 #pragma unroll
     for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
       const int64_t g = g0 + u;
       const int64_t i = (g - tb) * d.len / G + threadIdx.x;
-      xv[u] = (g < se && i < d.len) ? glb(d.in)[i] : 0.0f;
+      xv[u] = (g < se && i < d.len) ? load_elem(d.in, i) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
       const int64_t g = g0 + u;
       const int64_t i = (g - tb) * d.len / G + threadIdx.x;
-      if (g < se && i < d.len) atomicAdd(&lh[rank_key(xv[u], sc, xm) >> 20], 1u);
+      if (g < se && i < d.len) atomicAdd(&lh[rank_key(xv[u], sc, xm, scale) >> 20], 1u);
     }
     __syncthreads();
     flush_hist(lh, shist + (size_t)t * HIST_BINS);
@@ -448,23 +529,23 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
       // the sampled elements of digit b as composites (the groups of k_sample,
       // U groups in flight at a time)
       const float sc = task_scaler(d);
+      const bool scale = task_scales(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-      const RADIK_AS1 float* __restrict__ in = glb(d.in);
       uint64_t a = ~0ull, o = 0ull;
       constexpr int U = 16;
       for (int64_t j0 = 0; j0 < G; j0 += U) {
-        float xv[U];
+        uint32_t xv[U];
         uint32_t ps[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int64_t i = (j0 + u) * d.len / G + threadIdx.x;
           const bool ok = j0 + u < G && i < d.len;
-          xv[u] = ok ? in[i] : 0.0f;
+          xv[u] = ok ? load_elem(d.in, i) : 0u;
           ps[u] = ok ? (uint32_t)i : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const uint32_t rk = rank_key(xv[u], sc, xm);
+          const uint32_t rk = rank_key(xv[u], sc, xm, scale);
           const bool hit = ps[u] != 0xFFFFFFFFu && (rk >> 20) == b;
           const uint64_t hm = ballot(hit);
           if (hm) {
@@ -644,45 +725,47 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
   auto write = [&](uint32_t slot, uint64_t v) {
     if (slot < slen) glb(spill)[slot] = v;
   };
-  auto one = [&](float x, uint32_t pos) {
-    const uint32_t rk = rk_of<SCALE>(x, sc, xm);
+  auto one = [&](uint32_t x, uint32_t pos) {
+    const uint32_t rk = rank_bits<SCALE>(x, sc, xm);
     atomicAdd(&lh[rk >> 20], 1u);
     if (SPILL) stage_append(st, spills(rk, pos), ((uint64_t)rk << 32) | pos, ctr, write);
   };
   // Scalar head up to 128-byte alignment and tail, as in stream_range; the
-  // float4 body spills per iteration: the UNR*4 hits of each lane (a bit
+  // vector body spills per iteration: the UNR*EPV hits of each lane (a bit
   // mask) enter the wave's stage together, placed by one ballot per element
   // slot, so a rare spill costs a few scalar instructions per iteration.
+  constexpr int NB = EPV * UNR;  // elements per lane per iteration (<= 32)
+  static_assert(NB <= 32, "hit mask");
   const int64_t tid = threadIdx.x;
-  const RADIK_AS1 float* __restrict__ p = glb(d.in);
   int64_t a = s;
   const int64_t head_end = aligned_head_end(d.in, a, e);
-  for (int64_t i = a + tid; i < head_end; i += BLOCK) one(p[i], (uint32_t)i);
+  for (int64_t i = a + tid; i < head_end; i += BLOCK) one(load_elem(d.in, i), (uint32_t)i);
   a = head_end;
-  const int64_t nvec = (e - a) >> 2;
-  const RADIK_AS1 f32x4* __restrict__ pv = reinterpret_cast<const RADIK_AS1 f32x4*>(p + a);
+  const int64_t nvec = (e - a) / EPV;
+  const RADIK_AS1 u32x4* __restrict__ pv =
+      reinterpret_cast<const RADIK_AS1 u32x4*>(glb(reinterpret_cast<const char*>(d.in)) + a * EB);
   int64_t v = tid;
   for (; v + (int64_t)(UNR - 1) * BLOCK < nvec; v += (int64_t)UNR * BLOCK) {
-    f32x4 r[UNR];
+    u32x4 r[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) r[u] = __builtin_nontemporal_load(pv + v + (int64_t)u * BLOCK);
     uint32_t hit = 0;
-    const uint32_t p0 = (uint32_t)(a + (v << 2));  // position of this lane's first element (POS)
+    const uint32_t p0 = (uint32_t)(a + v * EPV);  // position of this lane's first element (POS)
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t rk = rk_of<SCALE>(r[u][j], sc, xm);
+      for (int j = 0; j < EPV; ++j) {
+        const uint32_t rk = rank_bits<SCALE>(vec_elem(r[u], j), sc, xm);
         atomicAdd(&lh[rk >> 20], 1u);
-        if (SPILL && spills(rk, p0 + (uint32_t)(u * BLOCK * 4 + j))) hit |= 1u << (4 * u + j);
+        if (SPILL && spills(rk, p0 + (uint32_t)(u * BLOCK * EPV + j))) hit |= 1u << (EPV * u + j);
       }
     }
     if (SPILL && ballot(hit != 0)) {
       {
-        uint64_t mk[4 * UNR];
+        uint64_t mk[NB];
         uint32_t total = 0;
 #pragma unroll
-        for (int b = 0; b < 4 * UNR; ++b) {
+        for (int b = 0; b < NB; ++b) {
           mk[b] = ballot((hit >> b) & 1u);
           total += (uint32_t)__builtin_popcountll(mk[b]);
         }
@@ -704,11 +787,11 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
           off = n0;
         }
 #pragma unroll
-        for (int b = 0; b < 4 * UNR; ++b) {
+        for (int b = 0; b < NB; ++b) {
           if ((hit >> b) & 1u) {
-            const int u = b >> 2, j = b & 3;
-            const uint32_t pos = (uint32_t)(a + ((v + (int64_t)u * BLOCK) << 2)) + (uint32_t)j;
-            const uint64_t ent = ((uint64_t)rk_of<SCALE>(r[u][j], sc, xm) << 32) | pos;
+            const int u = b / EPV, j = b % EPV;
+            const uint32_t pos = (uint32_t)(a + (v + (int64_t)u * BLOCK) * EPV) + (uint32_t)j;
+            const uint64_t ent = ((uint64_t)rank_bits<SCALE>(vec_elem(r[u], j), sc, xm) << 32) | pos;
             const uint32_t slot = off + lanes_below(mk[b]);
             if (direct) write(slot, ent);
             else st.buf[slot] = ent;
@@ -724,14 +807,12 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
     }
   }
   for (; v < nvec; v += BLOCK) {
-    const f32x4 r = __builtin_nontemporal_load(pv + v);
-    const uint32_t q = (uint32_t)(a + (v << 2));
-    one(r.x, q);
-    one(r.y, q + 1);
-    one(r.z, q + 2);
-    one(r.w, q + 3);
+    const u32x4 r = __builtin_nontemporal_load(pv + v);
+    const uint32_t q = (uint32_t)(a + v * EPV);
+#pragma unroll
+    for (int j = 0; j < EPV; ++j) one(vec_elem(r, j), q + (uint32_t)j);
   }
-  for (int64_t i = a + (nvec << 2) + tid; i < e; i += BLOCK) one(p[i], (uint32_t)i);
+  for (int64_t i = a + nvec * EPV + tid; i < e; i += BLOCK) one(load_elem(d.in, i), (uint32_t)i);
   if (SPILL) stage_flush(st, ctr, write);
 }
 
@@ -768,7 +849,7 @@ __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, c
     const float sc = task_scaler(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
     const uint32_t bp = ctl[t].b_pred, rp = ctl[t].r_pred, pp = ctl[t].p_pred;
-    const bool scale = (d.flags & F_WITHSCALE) != 0;
+    const bool scale = task_scales(d);
     uint32_t* ctr = &ctl[t].cand_ctr;
     uint64_t* spill = cand + d.base;
     const int64_t s = g0 - tb, e = se - tb;
@@ -861,18 +942,18 @@ __device__ __forceinline__ void pass_b_segment(const TaskDesc& d, int64_t s, int
   auto write_win = [&](uint32_t slot, uint64_t v) {
     if (slot < nwin) {
       if (fin) cb[slot] = v;
-      else put_result(d, slot, __uint_as_float((uint32_t)(v >> 32)), (uint32_t)v);
+      else put_result(d, slot, (uint32_t)(v >> 32), (uint32_t)v);
     }
   };
   auto write_cand = [&](uint32_t slot, uint64_t v) {
     if (slot < clen) cdst[slot] = v;
   };
-  stream_range<UNR>(d.in, s, e, [&](float x, uint32_t pos) {
-    const uint32_t rk = rk_of<SCALE>(x, sc, xm);
+  stream_range<UNR>(d.in, s, e, [&](uint32_t x, uint32_t pos) {
+    const uint32_t rk = rank_bits<SCALE>(x, sc, xm);
     const bool hit = rk <= rmax;
     if (ballot(hit)) {
       const uint32_t dg = rk >> 20;
-      const uint64_t wv = ((uint64_t)(fin ? rk : __float_as_uint(x)) << 32) | pos;
+      const uint64_t wv = ((uint64_t)(fin ? rk : x) << 32) | pos;
       stage_append(sw, hit && dg < bw, wv, wctr, write_win);
       stage_append(scd, hit && dg == bc, ((uint64_t)rk << 32) | pos, cctr, write_cand);
     }
@@ -911,7 +992,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
       uint32_t* wctr = &ctl[t].win_ctr;
       uint32_t* cctr = &ctl[t].cand_ctr;
       uint64_t* cb = cand + d.base;
-      if (d.flags & F_WITHSCALE) pass_b_segment<UNR, true>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
+      if (task_scales(d)) pass_b_segment<UNR, true>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
       else pass_b_segment<UNR, false>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
     }
     if (se >= g1) break;
@@ -1053,13 +1134,14 @@ __device__ __forceinline__ int select_digits(Src src, uint32_t n, int top, uint6
 // (in LDS when they fit) and emitting the chosen ones.
 // ---------------------------------------------------------------------------
 
-// Value of a spilled composite: the original element.
-__device__ __forceinline__ float spilled_value(const TaskDesc& d, uint64_t x, uint32_t& pos) {
+// Value of a spilled composite: the original element's bits (read back
+// from the input for scaled tasks, else inverted from the rank key).
+__device__ __forceinline__ uint32_t spilled_value(const TaskDesc& d, uint64_t x, uint32_t& pos) {
   pos = (uint32_t)x;
   if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
-  if (d.flags & F_WITHSCALE) return glb(d.in)[pos];
+  if (task_scales(d)) return load_elem(d.in, pos);
   const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-  return fkey_inv((uint32_t)(x >> 32) ^ xm);
+  return ekey_inv((uint32_t)(x >> 32) ^ xm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1100,10 +1182,10 @@ __device__ __forceinline__ void bitonic_sort_asc(Ptr a, uint32_t n) {
   }
 }
 
-// Final order key of a result: (ASCEND ? key : ~key) << 32 | position.
-__device__ __forceinline__ uint64_t out_composite(const TaskDesc& d, float v, uint32_t pos) {
+// Final order key of a result (value bits v): (ASCEND ? key : ~key) << 32 | position.
+__device__ __forceinline__ uint64_t out_composite(const TaskDesc& d, uint32_t v, uint32_t pos) {
   const uint32_t am = (d.flags & F_ASCEND) ? 0u : 0xFFFFFFFFu;
-  return ((uint64_t)(fkey(v) ^ am) << 32) | pos;
+  return ((uint64_t)(ekey(v) ^ am) << 32) | pos;
 }
 
 // Writes the k output slots of task d from sorted composites (padding past m).
@@ -1113,18 +1195,18 @@ __device__ __forceinline__ void write_sorted(const TaskDesc& d, Src s, uint32_t 
   const bool withidx = (d.flags & F_WITHIDXIN) != 0;
   const bool i64 = (d.flags & F_IDX64) != 0;
   for (uint32_t i = threadIdx.x; i < (uint32_t)d.k; i += BLOCK) {
-    float v = 0.0f;
+    uint32_t v = 0u;
     int64_t id = -1;
     if (i < m) {
       const uint64_t x = s[i];
       uint32_t pos = (uint32_t)x;
       if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
-      v = fkey_inv((uint32_t)(x >> 32) ^ am);
+      v = ekey_inv((uint32_t)(x >> 32) ^ am);
       if (withidx) id = i64 ? glb(reinterpret_cast<const int64_t*>(d.idx_in))[pos]
                             : (int64_t)glb(reinterpret_cast<const int32_t*>(d.idx_in))[pos];
       else id = (int64_t)pos + 1;
     }
-    glb(d.val_out)[i] = v;
+    store_elem(d.val_out, i, v);
     if (i64) glb(reinterpret_cast<int64_t*>(d.idx_out))[i] = id;
     else glb(reinterpret_cast<int32_t*>(d.idx_out))[i] = (int32_t)id;
   }
@@ -1194,21 +1276,21 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&a)[E], uint64_t* xs) {
 
 // One output slot from a sorted composite (or padding when x is the +inf pad).
 __device__ __forceinline__ void write_slot(const TaskDesc& d, uint32_t i, uint64_t x, bool valid) {
-  float v = 0.0f;
+  uint32_t v = 0u;
   int64_t id = -1;
   const bool i64 = (d.flags & F_IDX64) != 0;
   if (valid) {
     const uint32_t am = (d.flags & F_ASCEND) ? 0u : 0xFFFFFFFFu;
     uint32_t pos = (uint32_t)x;
     if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
-    v = fkey_inv((uint32_t)(x >> 32) ^ am);
+    v = ekey_inv((uint32_t)(x >> 32) ^ am);
     if (d.flags & F_WITHIDXIN)
       id = i64 ? glb(reinterpret_cast<const int64_t*>(d.idx_in))[pos]
                : (int64_t)glb(reinterpret_cast<const int32_t*>(d.idx_in))[pos];
     else
       id = (int64_t)pos + 1;
   }
-  glb(d.val_out)[i] = v;
+  store_elem(d.val_out, i, v);
   if (i64) glb(reinterpret_cast<int64_t*>(d.idx_out))[i] = id;
   else glb(reinterpret_cast<int32_t*>(d.idx_out))[i] = (int32_t)id;
 }
@@ -1224,7 +1306,7 @@ __device__ __forceinline__ void sort_task_regs(const TaskDesc& d, uint32_t m, ui
     if (i < m) {
       const uint32_t pos = i64 ? (uint32_t)glb(reinterpret_cast<const int64_t*>(d.idx_out))[i]
                                : (uint32_t)glb(reinterpret_cast<const int32_t*>(d.idx_out))[i];
-      a[l] = out_composite(d, glb(d.val_out)[i], pos);
+      a[l] = out_composite(d, load_elem(d.val_out, i), pos);
     } else {
       a[l] = ~0ull;
     }
@@ -1242,7 +1324,7 @@ __device__ __forceinline__ void sort_task_regs(const TaskDesc& d, uint32_t m, ui
 // Output composite of a selection composite (rank key << 32 | position).
 __device__ __forceinline__ uint64_t out_of_selection(const TaskDesc& d, uint64_t x) {
   uint32_t pos;
-  const float v = spilled_value(d, x, pos);
+  const uint32_t v = spilled_value(d, x, pos);
   return out_composite(d, v, pos);
 }
 
@@ -1250,7 +1332,7 @@ __device__ __forceinline__ uint64_t out_of_selection(const TaskDesc& d, uint64_t
 // output, or smallest with ascending output): then the composites are equal.
 __device__ __forceinline__ bool same_order(const TaskDesc& d) {
   const bool largest = (d.flags & F_LARGEST) != 0, ascend = (d.flags & F_ASCEND) != 0;
-  return !(d.flags & F_WITHSCALE) && largest != ascend;
+  return !task_scales(d) && largest != ascend;
 }
 
 // Sorts the m selection composites sel[0, m) (LDS) by output order in
@@ -1327,7 +1409,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
     for (uint32_t i = threadIdx.x; i < m; i += BLOCK) {
       const uint32_t pos = i64 ? (uint32_t)glb(reinterpret_cast<const int64_t*>(d.idx_out))[i]
                                : (uint32_t)glb(reinterpret_cast<const int32_t*>(d.idx_out))[i];
-      g[i] = out_composite(d, glb(d.val_out)[i], pos);
+      g[i] = out_composite(d, load_elem(d.val_out, i), pos);
     }
     __threadfence_block();
     __syncthreads();
@@ -1346,8 +1428,6 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
 // other run (binary searches, keys are distinct) and writes the outputs.
 // cbases: prefix sums of the runs per task; mbases: of the results per task.
 // ---------------------------------------------------------------------------
-constexpr uint32_t SORT_CHUNK = BLOCK * 8;
-constexpr uint32_t SORT_MAX_RUNS = 64;
 
 __global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restrict__ tasks,
                                                        const TaskCtl* __restrict__ ctl,
@@ -1366,7 +1446,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restric
     if (i < m) {
       const uint32_t pos = i64 ? (uint32_t)glb(reinterpret_cast<const int64_t*>(d.idx_out))[i]
                                : (uint32_t)glb(reinterpret_cast<const int32_t*>(d.idx_out))[i];
-      a[l] = out_composite(d, glb(d.val_out)[i], pos);
+      a[l] = out_composite(d, load_elem(d.val_out, i), pos);
     } else {
       a[l] = ~0ull;
     }
@@ -1526,7 +1606,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
           v[slot] = x;
         } else {
           uint32_t pos;
-          const float val = spilled_value(d, x, pos);
+          const uint32_t val = spilled_value(d, x, pos);
           put_result(d, slot, val, pos);
         }
       }
@@ -1617,7 +1697,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         const uint32_t slot = wave_append(w, wm, &wctr);
         if (w && slot < nwin) {
           uint32_t pos;
-          const float val = spilled_value(d, x, pos);
+          const uint32_t val = spilled_value(d, x, pos);
           put_result(d, slot, val, pos);
         }
       }
@@ -1626,7 +1706,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         const uint32_t slot = wave_append(lo, lm, &ectr);
         if (lo && slot < krem) {
           uint32_t pos;
-          const float val = spilled_value(d, x, pos);
+          const uint32_t val = spilled_value(d, x, pos);
           put_result(d, nwin + slot, val, pos);
         }
       }
@@ -1660,7 +1740,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
             const uint32_t slot = wave_append(tk, tm, &ectr);  // continues after the `below` ones
             if (tk && slot < krem) {
               uint32_t pos;
-              const float val = spilled_value(d, x, pos);
+              const uint32_t val = spilled_value(d, x, pos);
               put_result(d, nwin + slot, val, pos);
             }
           }
@@ -1704,10 +1784,11 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
     });
   } else {
     const float sc = task_scaler(d);
+    const bool scale = task_scales(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
     const uint32_t rmax = digit_bound(bc != NO_BIN ? bc : bw - 1);
-    stream_range<FIN_UNR>(d.in, 0, d.len, [&](float x, uint32_t pos) {
-      const uint32_t rk = rank_key(x, sc, xm);
+    stream_range<FIN_UNR>(d.in, 0, d.len, [&](uint32_t x, uint32_t pos) {
+      const uint32_t rk = rank_key(x, sc, xm, scale);
       const bool hit = rk <= rmax;
       if (ballot(hit)) {
         const uint32_t dg = rk >> 20;
@@ -1771,7 +1852,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
         const uint32_t slot = wave_append(tk, tm, &ectr);
         if (tk && slot < krem) {
           uint32_t pos;
-          const float val = spilled_value(d, x, pos);
+          const uint32_t val = spilled_value(d, x, pos);
           put_result(d, nwin + slot, val, pos);
         }
       }
@@ -1873,7 +1954,7 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
             wdst[ws] = xs[u];
           } else {
             uint32_t pos;
-            const float v = spilled_value(d, xs[u], pos);
+            const uint32_t v = spilled_value(d, xs[u], pos);
             put_result(d, ws, v, pos);
           }
         }
@@ -1903,8 +1984,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
   for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
   __syncthreads();
   const float sc = task_scaler(d);
+  const bool scale = task_scales(d);
   const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-  stream_range<FIN_UNR>(d.in, 0, d.len, [&](float x, uint32_t) { atomicAdd(&lh[rank_key(x, sc, xm) >> 20], 1u); });
+  stream_range<FIN_UNR>(d.in, 0, d.len,
+                        [&](uint32_t x, uint32_t) { atomicAdd(&lh[rank_key(x, sc, xm, scale) >> 20], 1u); });
   __syncthreads();
   uint32_t h[16];
 #pragma unroll
@@ -1933,6 +2016,7 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
   uint64_t* __restrict__ sel = sm + cap;
   if (n > cap) return;  // host routes only short tasks here
   const float sc = task_scaler(d);
+  const bool scale = task_scales(d);
   const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
   if (threadIdx.x == 0) {
     ectr = 0;
@@ -1942,17 +2026,17 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
   __syncthreads();
   uint64_t a = ~0ull, o = 0ull;
   for (uint32_t b = threadIdx.x; b < n; b += BLOCK * LOADS_IN_FLIGHT) {
-    float xs[LOADS_IN_FLIGHT];
+    uint32_t xs[LOADS_IN_FLIGHT];
 #pragma unroll
     for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
       const uint32_t i = b + (uint32_t)u * BLOCK;
-      xs[u] = (i < n) ? glb(d.in)[i] : 0.0f;
+      xs[u] = (i < n) ? load_elem(d.in, i) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < LOADS_IN_FLIGHT; ++u) {
       const uint32_t i = b + (uint32_t)u * BLOCK;
       if (i < n) {
-        const uint64_t x = ((uint64_t)rank_key(xs[u], sc, xm) << 32) | i;
+        const uint64_t x = ((uint64_t)rank_key(xs[u], sc, xm, scale) << 32) | i;
         all[i] = x;
         a &= x;
         o |= x;
@@ -1964,7 +2048,6 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
   uint64_t P = 0;
   if (m > 0 && m < n)
     s = select_digits(LdsSrc{all}, n, 64, P, m, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh, wsum, res);
-  const bool scale = (d.flags & F_WITHSCALE) != 0;
   if (m > 0) {
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
       const uint64_t x = all[i];
@@ -1974,7 +2057,7 @@ __global__ __launch_bounds__(BLOCK) void k_small(const TaskDesc* __restrict__ ta
         const uint32_t slot = wave_append(take, tm, &ectr);
         if (take && slot < m) {
           const uint32_t pos = (uint32_t)x;
-          const float v = scale ? glb(d.in)[pos] : fkey_inv((uint32_t)(x >> 32) ^ xm);
+          const uint32_t v = scale ? load_elem(d.in, pos) : ekey_inv((uint32_t)(x >> 32) ^ xm);
           sel[slot] = out_composite(d, v, pos);
         }
       }
@@ -2002,15 +2085,6 @@ __global__ __launch_bounds__(BLOCK) void k_diag_scan(const uint32_t* __restrict_
   }
 }
 
-// Zero fill of two ranges of 32-bit words (the histograms of a pipeline that
-// lives in fixed storage, cleared at the start of each of its launches).
-__global__ __launch_bounds__(BLOCK) void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restrict__ b,
-                                                 int64_t nb) {
-  const int64_t stride = (int64_t)gridDim.x * BLOCK;
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < na; i += stride) a[i] = 0u;
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nb; i += stride) b[i] = 0u;
-}
-
 // Plain streaming copy (16-byte loads and stores), the reference point for
 // HBM traffic counters: it reads and writes exactly `n16` * 16 bytes.
 __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst,
@@ -2019,37 +2093,5 @@ __global__ __launch_bounds__(BLOCK) void k_diag_copy(const f32x4* __restrict__ s
     glb(dst)[i] = __builtin_nontemporal_load(glb(src) + i);
 }
 
-// Explicit instantiations used by the host.
-#define RADIK_INST(U)                                                                                          \
-  template __global__ void k_hist<U, true, true>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,   \
-                                                 TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                   \
-  template __global__ void k_hist<U, true, false>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*,  \
-                                                  TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                  \
-  template __global__ void k_hist<U, false, false>(const TaskDesc*, const int64_t*, int, const int64_t*, uint32_t*, \
-                                                   TaskCtl*, uint64_t*, const uint32_t*, uint32_t);                 \
-  template __global__ void k_compact<U>(const TaskDesc*, const int64_t*, int, const int64_t*, TaskCtl*,             \
-                                        uint64_t*, const uint32_t*, int64_t);
-RADIK_INST(1)
-RADIK_INST(2)
-RADIK_INST(4)
-#undef RADIK_INST
-template __global__ void k_medium<1>(const TaskDesc*, uint64_t*, uint32_t);
-template __global__ void k_medium<2>(const TaskDesc*, uint64_t*, uint32_t);
-template __global__ void k_medium<4>(const TaskDesc*, uint64_t*, uint32_t);
-template __global__ void k_medium<8>(const TaskDesc*, uint64_t*, uint32_t);
-template __global__ void k_finish<1>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
-template __global__ void k_finish<2>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
-template __global__ void k_finish<4>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
-template __global__ void k_finish<8>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
-template __global__ void k_finish<16>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
-template __global__ void k_finish<32>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
-template __global__ void k_finish<64>(const TaskDesc*, TaskCtl*, uint32_t*, uint64_t*, uint32_t, uint32_t, int64_t, int, int);
-template __global__ void k_sort<1>(const TaskDesc*, const TaskCtl*, uint64_t*);
-template __global__ void k_sort<2>(const TaskDesc*, const TaskCtl*, uint64_t*);
-template __global__ void k_sort<4>(const TaskDesc*, const TaskCtl*, uint64_t*);
-template __global__ void k_sort<8>(const TaskDesc*, const TaskCtl*, uint64_t*);
-template __global__ void k_sort<16>(const TaskDesc*, const TaskCtl*, uint64_t*);
-template __global__ void k_sort<32>(const TaskDesc*, const TaskCtl*, uint64_t*);
-template __global__ void k_sort<64>(const TaskDesc*, const TaskCtl*, uint64_t*);
-
+}  // namespace RADIK_KNS
 }  // namespace radik

@@ -10,7 +10,6 @@
 
 #include "../../include/radik.h"
 #include "radik_host.h"
-#include "radik_kernels.hip"
 
 namespace radik {
 
@@ -36,6 +35,16 @@ const char* last_error() { return g_err.c_str(); }
 static std::mutex g_facts_mu;
 static DevFacts g_facts[64];
 
+// The kernels of each key type (radik_launch.hip, one object per type).
+#define RADIK_DECL_KT(NS)                                                                            \
+  int launch_setup_##NS(size_t lds_budget, int* hist_occ);                                          \
+  int enqueue_##NS(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm);
+RADIK_DECL_KT(kf32)
+RADIK_DECL_KT(kf16)
+RADIK_DECL_KT(kbf16)
+RADIK_DECL_KT(ki32)
+#undef RADIK_DECL_KT
+
 int dev_facts(DevFacts** out) {
   int dev = 0;
   RADIK_HIP(hipGetDevice(&dev));
@@ -47,39 +56,18 @@ int dev_facts(DevFacts** out) {
     RADIK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     RADIK_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     f.cus = cus > 0 ? cus : 256;
-    // LDS budget per block: the device's limit, at most 64 KiB until larger
-    // dynamic allocations are enabled per kernel below.
+    // LDS budget per block: the device's limit once every key type's
+    // per-task kernels are opted into the larger allocation, else 64 KiB.
     size_t budget = (size_t)(lds > 0 ? lds : 65536);
-    if (budget > 65536) {
-      // Opt the per-task kernels into the larger allocation.
-      const int want = (int)budget;
-      const int fin_want = want - (int)FINISH_STATIC_LDS;
-      bool ok = hipFuncSetAttribute((const void*)k_finish<1>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_finish<2>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_finish<4>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_finish<8>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_finish<16>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_finish<32>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_finish<64>, hipFuncAttributeMaxDynamicSharedMemorySize, fin_want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_sort<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_sort<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    want) == hipSuccess &&
-                hipFuncSetAttribute((const void*)k_small, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    want - 20 * 1024) == hipSuccess;
-      if (!ok) {
-        (void)hipGetLastError();
-        budget = 65536;
-      }
+    int occ[NUM_KEY_TYPES] = {4, 4, 4, 4};
+    const bool ok = launch_setup_kf32(budget, &occ[KEY_F32]) == 0 && launch_setup_kf16(budget, &occ[KEY_F16]) == 0 &&
+                    launch_setup_kbf16(budget, &occ[KEY_BF16]) == 0 && launch_setup_ki32(budget, &occ[KEY_I32]) == 0;
+    if (!ok) {
+      (void)hipGetLastError();
+      budget = 65536;
     }
     f.lds_per_block = budget;
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_hist<4, true, true>, BLOCK, 0) != hipSuccess ||
-        occ < 1) {
-      (void)hipGetLastError();
-      occ = 4;
-    }
-    f.hist_blocks_per_cu = occ;
+    for (int t = 0; t < NUM_KEY_TYPES; ++t) f.hist_blocks_per_cu[t] = occ[t] > 0 ? occ[t] : 4;
     const size_t stat = 20 * 1024;  // static LDS of k_small (histogram + scratch)
     f.refine_cap_max = (uint32_t)((budget - FINISH_STATIC_LDS) / 8);
     f.small_cap_max = (uint32_t)((budget - stat) / 16);
@@ -94,6 +82,16 @@ int dev_facts(DevFacts** out) {
   }
   *out = &f;
   return 0;
+}
+
+int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm) {
+  switch (o.key_type) {
+    case KEY_F32: return enqueue_kf32(p, f, o, s, tm);
+    case KEY_F16: return enqueue_kf16(p, f, o, s, tm);
+    case KEY_BF16: return enqueue_kbf16(p, f, o, s, tm);
+    case KEY_I32: return enqueue_ki32(p, f, o, s, tm);
+    default: return set_error(1, "unknown key type %d", o.key_type);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -335,11 +333,11 @@ struct ChunkShape {
 
 static ChunkShape chunk_shape(const DevFacts& f, const LaunchOpts& o) {
   ChunkShape c;
-  const int res_cu = std::max(1, f.hist_blocks_per_cu);
+  const int res_cu = std::max(1, f.hist_blocks_per_cu[o.key_type]);
   c.guided = o.blocks_per_cu > res_cu;
   c.slots = (int64_t)f.cus * (c.guided ? res_cu : std::max(1, o.blocks_per_cu));
   c.spread = c.guided ? std::max(1, o.blocks_per_cu / (2 * res_cu)) : 1;
-  c.gran = (int64_t)BLOCK * 4 * o.unr;
+  c.gran = (int64_t)BLOCK * (16 / key_bytes(o.key_type)) * o.unr;
   c.min_chunk = (c.guided ? 8 : 4) * c.gran;
   return c;
 }
@@ -487,215 +485,6 @@ int upload(Pipeline& p, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Kernel sequence
-// ---------------------------------------------------------------------------
-// Pass A with the fused selection (fin_cap: see k_finish).
-template <int UNR>
-static void launch_stream_kernels_hist(const Pipeline& p, uint32_t fin_cap, hipStream_t s) {
-  if (p.grid_h <= 0) return;
-  if (p.pos_bounds)
-    hipLaunchKernelGGL((k_hist<UNR, true, true>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                       (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
-  else
-    hipLaunchKernelGGL((k_hist<UNR, true, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                       (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, fin_cap);
-}
-
-// Pass B over the whole element space (pass A's chunks, one per block);
-// blocks skip tasks of at most BIG_LEN elements and those with a usable spill.
-template <int UNR>
-static void launch_stream_kernels_compact(const Pipeline& p, hipStream_t s) {
-  if (p.n_chunks <= 0) return;
-  hipLaunchKernelGGL(k_compact<UNR>, dim3((unsigned)p.n_chunks), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                     (int)p.n_large, p.d_cstart, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN);
-}
-
-// k_small and k_medium (independent of the long tasks).
-static int launch_short_and_medium(const Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s) {
-  if (p.n_small) {
-    const uint32_t scap = eff_small_cap(f, o);
-    hipLaunchKernelGGL(k_small, dim3((unsigned)p.n_small), dim3(BLOCK), (size_t)scap * 16, s, p.d_small, scap);
-    RADIK_HIP(hipGetLastError());
-  }
-  if (p.n_med) {
-    // (at least 1/(MED_MAX_PER_CU + 1) of a CU's LDS per block, so a CU keeps
-    // room for the sampling and prediction kernels beside the medium tasks)
-    const size_t lds = std::max<size_t>(((size_t)MED_CAP + std::min<uint32_t>(p.max_m_med, MED_CAP)) * 8,
-                                        f.lds_per_block / (MED_MAX_PER_CU + 1) + 256);
-    const dim3 g((unsigned)p.n_med), b(BLOCK);
-    if (p.max_m_med <= (uint32_t)BLOCK) hipLaunchKernelGGL(k_medium<1>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
-    else if (p.max_m_med <= 2u * BLOCK) hipLaunchKernelGGL(k_medium<2>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
-    else if (p.max_m_med <= 4u * BLOCK) hipLaunchKernelGGL(k_medium<4>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
-    else hipLaunchKernelGGL(k_medium<8>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
-    RADIK_HIP(hipGetLastError());
-  }
-  return 0;
-}
-
-int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm) {
-  const bool timed = tm && tm->on;
-  int phase = 0;
-  auto mark = [&]() -> int {  // closes the current phase
-    if (timed) RADIK_HIP(hipEventRecord(tm->ev[phase], s));
-    ++phase;
-    return 0;
-  };
-  int rc;
-  if ((rc = mark())) return rc;  // start
-  // Short and medium tasks are independent of the long ones: outside the
-  // phase timing they run on the side stream, overlapping the sampling and
-  // pass A.
-  const bool fork = !timed && (p.n_small || p.n_med) && p.n_large && p.side;
-  if (fork) {
-    RADIK_HIP(hipEventRecord(p.ev_fork, s));
-    RADIK_HIP(hipStreamWaitEvent(p.side, p.ev_fork, 0));
-    if ((rc = launch_short_and_medium(p, f, o, p.side))) return rc;
-    RADIK_HIP(hipEventRecord(p.ev_join, p.side));
-  }
-  if (p.n_large) {
-    if (p.total_groups > 0) {
-      hipLaunchKernelGGL(k_sample, dim3(p.grid_s), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
-                         p.total_groups, p.d_shist, p.d_bmap + p.off_s);
-      RADIK_HIP(hipGetLastError());
-    }
-    if ((rc = mark())) return rc;
-    hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
-                       p.d_ctl);
-    RADIK_HIP(hipGetLastError());
-    if ((rc = mark())) return rc;
-    // k_finish's LDS: cap entries for a task's winners + candidates (a power
-    // of two >= 2048, which also holds the 4096-bin histogram of the wide
-    // digits), plus the m results of fin tasks; results sorted in k_finish
-    // when every task's m fits cap (else k_sort follows, without fin tasks).
-    const uint32_t budget = (uint32_t)((f.lds_per_block - FINISH_STATIC_LDS) / 8);
-    uint32_t cap = 2048;
-    {
-      const uint32_t want = (p.n_large <= (int64_t)f.cus) ? 16384u : std::max<uint32_t>(2048u, eff_refine_cap(f, o));
-      while (cap * 2 <= want && cap * 2 <= budget) cap *= 2;
-    }
-    bool sort_here = false;
-    uint32_t sel = 0;
-    for (;;) {
-      sort_here = p.max_m <= cap && p.max_m <= (uint32_t)BLOCK * 64;
-      sel = sort_here ? std::min<uint32_t>(p.max_m, cap) : 0u;
-      if (cap + sel <= budget || cap <= 2048) break;
-      cap /= 2;
-    }
-    if (cap + sel > budget) {
-      sort_here = false;
-      sel = 0;
-    }
-    const uint32_t fin_cap = sort_here ? cap : 0u;
-    uint32_t emax = 1;
-    while (sort_here && (uint32_t)BLOCK * emax < p.max_m) emax *= 2;
-    switch (o.unr) {
-      case 1: launch_stream_kernels_hist<1>(p, fin_cap, s); break;
-      case 4: launch_stream_kernels_hist<4>(p, fin_cap, s); break;
-      default: launch_stream_kernels_hist<2>(p, fin_cap, s); break;
-    }
-    RADIK_HIP(hipGetLastError());
-    if ((rc = mark())) return rc;
-    // Tasks spread over several pass-A blocks are selected by k_finish, or
-    // before pass B over several blocks (needed only for long tasks without a
-    // usable spill; k_finish re-reads the shorter ones).
-    if (p.has_big) {
-      hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl,
-                         fin_cap);
-      RADIK_HIP(hipGetLastError());
-    }
-    if ((rc = mark())) return rc;
-    if (p.has_big) {
-      switch (o.unr) {
-        case 1: launch_stream_kernels_compact<1>(p, s); break;
-        case 4: launch_stream_kernels_compact<4>(p, s); break;
-        default: launch_stream_kernels_compact<2>(p, s); break;
-      }
-      RADIK_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_split, dim3((unsigned)p.n_split), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
-                         p.d_xmap);
-      RADIK_HIP(hipGetLastError());
-    }
-    if ((rc = mark())) return rc;
-    {
-      const size_t lds = ((size_t)cap + sel) * 8;
-      const dim3 g((unsigned)p.n_large), b(BLOCK);
-      const int sh = sort_here ? 1 : 0;
-      const int sp = p.has_big ? 1 : 0;  // k_split listed the long tasks
-      switch (emax) {
-        case 1: hipLaunchKernelGGL(k_finish<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 2: hipLaunchKernelGGL(k_finish<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 4: hipLaunchKernelGGL(k_finish<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 8: hipLaunchKernelGGL(k_finish<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 16: hipLaunchKernelGGL(k_finish<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 32: hipLaunchKernelGGL(k_finish<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        default: hipLaunchKernelGGL(k_finish<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-      }
-      RADIK_HIP(hipGetLastError());
-    }
-    if ((rc = mark())) return rc;
-    if (!sort_here) {
-      if (p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS) {
-        // several blocks per task: sorted runs, then ranks across runs
-        hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)p.total_runs), dim3(BLOCK), 0, s, p.d_large, p.d_ctl,
-                           p.d_cbases, (int)p.n_large, p.d_cand);
-        RADIK_HIP(hipGetLastError());
-        const unsigned rg = (unsigned)((p.total_m + BLOCK - 1) / BLOCK);
-        hipLaunchKernelGGL(k_sort_rank, dim3(rg), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_mbases,
-                           (int)p.n_large, p.total_m, (const uint64_t*)p.d_cand);
-      } else {
-        uint32_t e = 1;
-        while ((uint32_t)BLOCK * e < p.max_m && (uint32_t)BLOCK * e < f.sort_cap) e *= 2;
-        const size_t lds = (size_t)BLOCK * e * 8;
-        const dim3 g((unsigned)p.n_large), b(BLOCK);
-        switch (e) {
-          case 1: hipLaunchKernelGGL(k_sort<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-          case 2: hipLaunchKernelGGL(k_sort<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-          case 4: hipLaunchKernelGGL(k_sort<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-          case 8: hipLaunchKernelGGL(k_sort<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-          case 16: hipLaunchKernelGGL(k_sort<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-          case 32: hipLaunchKernelGGL(k_sort<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-          default: hipLaunchKernelGGL(k_sort<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_cand); break;
-        }
-      }
-      RADIK_HIP(hipGetLastError());
-    }
-  } else {
-    for (int i = 0; i < 6; ++i)
-      if ((rc = mark())) return rc;
-  }
-  if ((rc = mark())) return rc;  // end of sort
-  if (fork) {
-    RADIK_HIP(hipStreamWaitEvent(s, p.ev_join, 0));
-  } else if ((rc = launch_short_and_medium(p, f, o, s))) {
-    return rc;
-  }
-  if ((rc = mark())) return rc;  // end of small
-  return 0;
-}
-
-// Diagnostics entry points' kernels.
-int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStream_t s) {
-  if (n <= 0) return 0;
-  const int grid = (int)((n + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_diag_scan, dim3(grid), dim3(BLOCK), 0, s, in, ow, ob, n);
-  RADIK_HIP(hipGetLastError());
-  return 0;
-}
-
-int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
-  const int64_t n16 = bytes / 16;
-  if (n16 <= 0) return 0;
-  int dev = 0, cus = 0;
-  RADIK_HIP(hipGetDevice(&dev));
-  RADIK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int64_t want = (int64_t)(cus > 0 ? cus : 256) * 16;
-  const int grid = (int)std::min<int64_t>(want, (n16 + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_diag_copy, dim3(grid), dim3(BLOCK), 0, s, (const f32x4*)src, (f32x4*)dst, n16);
-  RADIK_HIP(hipGetLastError());
-  return 0;
-}
-
-// ---------------------------------------------------------------------------
 // Allocation-free launches (radik_topk_noalloc): the pipeline lives in
 // `scratch`, a file-scope device array (one copy per GPU, held until the
 // process ends), and a static host arena.  Limits: NOALLOC_MAX_LONG long
@@ -707,6 +496,15 @@ int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
 // before returning; work another caller enqueues on the same GPU's scratch
 // outside this function (none exists in this library) would race.
 // ---------------------------------------------------------------------------
+// Zero fill of two ranges of 32-bit words (the histograms of a pipeline that
+// lives in fixed storage, cleared at the start of each of its launches).
+__global__ __launch_bounds__(BLOCK) void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restrict__ b,
+                                                 int64_t nb) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < na; i += stride) a[i] = 0u;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nb; i += stride) b[i] = 0u;
+}
+
 constexpr int64_t NOALLOC_MAX_LONG = RADIK_NOALLOC_MAX_LONG;
 constexpr int64_t NOALLOC_MAX_ELEMS = RADIK_NOALLOC_MAX_ELEMS;
 constexpr int64_t NOALLOC_MAX_TASKS = RADIK_NOALLOC_MAX_TASKS;
@@ -778,27 +576,6 @@ int noalloc_run(float* val_out, void* idx_out, int idx_bytes, const float* val_i
   rc = enqueue(p, *f, o, s, nullptr);
   if (rc) return rc;
   RADIK_HIP(hipStreamSynchronize(s));
-  return 0;
-}
-
-int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s) {
-  if (!p.n_large) return 0;
-  // Control blocks without a prediction: pass A only counts.
-  // staged without prediction: k_predict only initialises the control blocks
-  hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
-                     p.d_ctl);
-  RADIK_HIP(hipGetLastError());
-  if (p.grid_h > 0) {
-    hipLaunchKernelGGL((k_hist<2, false, false>), dim3(p.grid_h), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                       (int)p.n_large, p.d_cstart, p.d_hist, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, 0u);
-    RADIK_HIP(hipGetLastError());
-  }
-  RADIK_HIP(hipMemcpyAsync(hist_out, p.d_hist, (size_t)p.n_large * HIST_BINS * sizeof(uint32_t),
-                           hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl, 0u);
-  RADIK_HIP(hipGetLastError());
-  RADIK_HIP(hipMemcpy2DAsync(sel_out, 4 * sizeof(uint32_t), p.d_ctl, sizeof(TaskCtl), 4 * sizeof(uint32_t),
-                             (size_t)p.n_large, hipMemcpyDeviceToDevice, s));
   return 0;
 }
 

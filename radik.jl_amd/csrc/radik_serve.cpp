@@ -30,9 +30,9 @@ namespace {
 
 struct Request {
   int64_t ticket;
-  const float* in;
+  const void* in;
   int64_t rows, cols, k;
-  float* val_out;
+  void* val_out;
   int32_t* idx_out;
   int64_t bytes;  // admission footprint
   Clock::time_point t_submit;
@@ -117,13 +117,15 @@ static void dispatch_loop(radik_server* s) {
     slot->specs.clear();
     slot->tickets.clear();
     const uint32_t tflags = s->flags & 0x0Fu & ~RADIK_WITHIDXIN;
+    const int kt = (int)((s->flags & RADIK_KEY_MASK) >> RADIK_KEY_SHIFT);
+    const int eb = key_bytes(kt);
     int64_t nrows = 0;
     for (const Request& r : batch) {
       for (int64_t i = 0; i < r.rows; ++i) {
         TaskSpec sp;
-        sp.in = r.in + i * r.cols;
+        sp.in = elem_ptr(r.in, i * r.cols, eb);
         sp.idx_in = nullptr;
-        sp.val_out = r.val_out + i * r.k;
+        sp.val_out = elem_ptr(r.val_out, i * r.k, eb);
         sp.idx_out = r.idx_out + i * r.k;
         sp.len = r.cols;
         sp.k = r.k;
@@ -134,6 +136,7 @@ static void dispatch_loop(radik_server* s) {
       slot->tickets.push_back(r.ticket);
     }
     LaunchOpts o;
+    o.key_type = kt;
     int rc = stage_tasks(slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, o, slot->stream);
     if (!rc) rc = upload(slot->pipe, slot->stream);
     if (!rc) rc = enqueue(slot->pipe, *f, o, slot->stream, nullptr);
@@ -202,6 +205,8 @@ int radik_server_create(radik_server** out, int64_t budget_bytes, int max_inflig
   *out = nullptr;
   if (budget_bytes <= 0 || max_inflight <= 0 || max_inflight > 16)
     return set_error(RADIK_EINVAL, "budget must be > 0 and max_inflight in 1..16");
+  if ((flags & RADIK_WITHSCALE) && (flags & RADIK_KEY_MASK))
+    return set_error(RADIK_EINVAL, "WITHSCALE needs Float32 keys");
   std::unique_ptr<radik_server> s(new (std::nothrow) radik_server());
   if (!s) return set_error(RADIK_ENOMEM, "out of host memory");
   RADIK_HIP(hipGetDevice(&s->device));
@@ -224,7 +229,7 @@ int radik_server_create(radik_server** out, int64_t budget_bytes, int max_inflig
   return 0;
 }
 
-int radik_server_submit(radik_server* s, const float* in, int64_t rows, int64_t cols, int64_t k, float* val_out,
+int radik_server_submit(radik_server* s, const void* in, int64_t rows, int64_t cols, int64_t k, void* val_out,
                         int32_t* idx_out, int64_t* ticket) {
   clear_error();
   if (!s || !ticket) return set_error(RADIK_EINVAL, "NULL argument");

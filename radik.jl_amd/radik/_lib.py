@@ -14,6 +14,7 @@ RADIK_ASCEND = 0x02
 RADIK_WITHSCALE = 0x04
 RADIK_WITHIDXIN = 0x08
 RADIK_WITHPACKING = 0x10
+RADIK_KEY_F32, RADIK_KEY_F16, RADIK_KEY_BF16, RADIK_KEY_I32 = 0x000, 0x100, 0x200, 0x300
 
 RADIK_OK = 0
 RADIK_EINVAL = 1
@@ -52,6 +53,7 @@ SIGNATURES = {
     "radik_ws_destroy": (_INT, [_P]),
     "radik_topk_radix_select": (_INT, [_P, _P, _P, _INT, _P, _P, _I64P, _I64, _I64, _U32, _INT, _P]),
     "radik_topk": (_INT, [_P, _P, _INT, _I64, _I64, _P, _P, _I64, _I64P, _I64, _I64, _INT, _INT, _P]),
+    "radik_topk_typed": (_INT, [_INT, _P, _P, _INT, _I64, _I64, _P, _P, _I64, _I64P, _I64, _I64, _INT, _INT, _P]),
     "radik_topk_noalloc": (_INT, [_P, _P, _INT, _I64, _I64, _P, _P, _I64, _I64P, _I64, _I64, _INT, _INT, _P]),
     "radik_plan_create": (_INT, [ctypes.POINTER(_P), _I64P, _I64P, _I64, _U32, _INT, _P]),
     "radik_plan_execute": (_INT, [_P, _P, _P, _P, _P, _P]),

@@ -64,6 +64,33 @@ def _flags(LARGEST, ASCEND, WITHSCALE, WITHIDXIN, WITHPACKING):
     return f
 
 
+KEYS = {"f32": 0, "f16": 1, "bf16": 2, "i32": 3}
+_KEY_OF_DTYPE = {np.dtype(np.float32): "f32", np.dtype(np.float16): "f16", np.dtype(np.int32): "i32"}
+
+
+def key_type(data, key=None):
+    """Key type code of `data`: float32, float16 and int32 arrays by their
+    dtype; bfloat16 data are uint16 bit patterns with key="bf16".  The
+    reference takes Float32 only (algorithm.jl:492-494) and raises for
+    anything else; here the other types run the same path on their own
+    order-preserving keys."""
+    if key is not None:
+        if key not in KEYS:
+            raise RadiKError(_lib.RADIK_EINVAL, f"unknown key type {key!r}; one of {sorted(KEYS)}")
+        if key == "bf16" and np.dtype(data.dtype) != np.uint16:
+            raise RadiKError(_lib.RADIK_EINVAL, "bfloat16 data are passed as uint16 bit patterns")
+        if key != "bf16" and _KEY_OF_DTYPE.get(np.dtype(data.dtype)) != key:
+            raise RadiKError(_lib.RADIK_EINVAL, f"data of dtype {data.dtype} cannot be read as {key}")
+        return KEYS[key]
+    k = _KEY_OF_DTYPE.get(np.dtype(data.dtype))
+    if k is None:
+        # algorithm.jl:492-494 (the reference raises MethodError earlier, Q10)
+        raise RadiKError(_lib.RADIK_EINVAL,
+                         f"RadiK supports Float32, Float16, BFloat16 (uint16 bits, key='bf16') and Int32 "
+                         f"input data. Got {data.dtype}.")
+    return KEYS[k]
+
+
 def _check_f32(data):
     if np.dtype(data.dtype) != np.float32:
         # algorithm.jl:492-494 (the reference raises MethodError earlier, Q10)
@@ -72,11 +99,15 @@ def _check_f32(data):
 
 
 def topk_radix_select_(val_out, idx_out, ws, val_in, idx_in, task_lens, k, LARGEST=True, ASCEND=True,
-                       WITHSCALE=False, WITHIDXIN=False, WITHPACKING=True, VLOAD_SIZE_BYTES=32, stream=None):
-    """Writes the top-k of every task into val_out/idx_out ([k, T]); returns them."""
-    _check_f32(val_in)
+                       WITHSCALE=False, WITHIDXIN=False, WITHPACKING=True, VLOAD_SIZE_BYTES=32, stream=None,
+                       key=None):
+    """Writes the top-k of every task into val_out/idx_out ([k, T]); returns them.
+    val_out has the element type of val_in (key: see key_type)."""
+    kt = key_type(val_in, key)
+    if np.dtype(val_out.dtype) != np.dtype(val_in.dtype):
+        raise RadiKError(_lib.RADIK_EINVAL, "val_out must have the element type of val_in")
     lens = [int(x) for x in task_lens]
-    flags = _flags(LARGEST, ASCEND, WITHSCALE, WITHIDXIN, WITHPACKING)
+    flags = _flags(LARGEST, ASCEND, WITHSCALE, WITHIDXIN, WITHPACKING) | (kt << 8)
     idx_bytes = np.dtype(idx_out.dtype).itemsize
     # the library sees pointers only: sizes are checked here
     n_in, n_out = sum(lens), int(k) * len(lens)
@@ -95,7 +126,7 @@ def topk_radix_select_(val_out, idx_out, ws, val_in, idx_in, task_lens, k, LARGE
     return val_out, idx_out
 
 
-def topk_(val_out, idx_out, data, *args, indices=None, largest=True, rev=False, stream=None):
+def topk_(val_out, idx_out, data, *args, indices=None, largest=True, rev=False, stream=None, key=None):
     """topk_(val_out, idx_out, data, [task_lens,] k; indices, largest, rev).
 
     Validates sizes (ErrorException messages of algorithm.jl:498-509) and
@@ -106,7 +137,9 @@ def topk_(val_out, idx_out, data, *args, indices=None, largest=True, rev=False, 
         task_lens, k = args
     else:
         raise TypeError("topk_(val_out, idx_out, data, [task_lens,] k)")
-    _check_f32(data)
+    kt = key_type(data, key)
+    if np.dtype(val_out.dtype) != np.dtype(data.dtype):
+        raise RadiKError(_lib.RADIK_EINVAL, "val_out must have the element type of data")
     lens = [int(x) for x in task_lens]
     T = len(lens)
     shape = val_out.shape if len(val_out.shape) == 2 else (val_out.shape[0], 1)
@@ -122,31 +155,31 @@ def topk_(val_out, idx_out, data, *args, indices=None, largest=True, rev=False, 
         if np.dtype(indices.dtype) != np.dtype(idx_out.dtype):
             raise TypeError("indices and idx_out must share their element type")
     idx_bytes = np.dtype(idx_out.dtype).itemsize
-    _lib.check(_lib.load().radik_topk(
-        val_out.ptr, idx_out.ptr, idx_bytes, shape[0], shape[1], data.ptr,
+    _lib.check(_lib.load().radik_topk_typed(
+        kt, val_out.ptr, idx_out.ptr, idx_bytes, shape[0], shape[1], data.ptr,
         indices.ptr if n_ind else None, n_ind, _lib.i64_array(lens), T, int(k),
         1 if largest else 0, 1 if rev else 0, _stream_handle(stream)))
     return val_out, idx_out
 
 
-def topk(data, *args, indices=None, largest=True, rev=False, stream=None):
+def topk(data, *args, indices=None, largest=True, rev=False, stream=None, key=None):
     """topk(data, k) -> (values[k], indices[k]);
     topk(data, task_lens, k) -> (values[k, T], indices[k, T]).
 
     Index element type: Int32, or that of `indices` (algorithm.jl:548)."""
     if len(args) == 1:
         k = int(args[0])
-        v, i = topk(data, [data.size], k, indices=indices, largest=largest, rev=rev, stream=stream)
+        v, i = topk(data, [data.size], k, indices=indices, largest=largest, rev=rev, stream=stream, key=key)
         return v.column(0), i.column(0)
     if len(args) != 2:
         raise TypeError("topk(data, [task_lens,] k)")
     task_lens, k = args
-    _check_f32(data)
+    key_type(data, key)
     T = len(task_lens)
     k = int(k)
     idx_type = np.int32 if indices is None else indices.dtype
-    val_out = DeviceArray.zeros((k, T), np.float32)
+    val_out = DeviceArray.zeros((k, T), data.dtype)
     idx_out = DeviceArray.zeros((k, T), idx_type)
-    topk_(val_out, idx_out, data, task_lens, k, indices=indices, largest=largest, rev=rev, stream=stream)
+    topk_(val_out, idx_out, data, task_lens, k, indices=indices, largest=largest, rev=rev, stream=stream, key=key)
     return val_out, idx_out
 

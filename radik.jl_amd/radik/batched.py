@@ -15,7 +15,7 @@ from .device import DeviceArray
 
 class Plan:
     def __init__(self, task_lens, k, largest=True, rev=False, withscale=False, indices_dtype=None,
-                 idx_dtype=np.int32, use_graph=True, stream=None, tuned=True):
+                 idx_dtype=np.int32, use_graph=True, stream=None, tuned=True, key="f32"):
         lens = [int(x) for x in task_lens]
         ks = [int(k)] * len(lens) if np.isscalar(k) else [int(x) for x in k]
         if len(ks) != len(lens):
@@ -28,6 +28,12 @@ class Plan:
         flags |= 0 if rev else _lib.RADIK_ASCEND
         flags |= _lib.RADIK_WITHSCALE if withscale else 0
         flags |= _lib.RADIK_WITHIDXIN if indices_dtype is not None else 0
+        from .api import KEYS
+        if key not in KEYS:
+            raise _lib.RadiKError(_lib.RADIK_EINVAL, f"unknown key type {key!r}")
+        self.key = key
+        self.val_dtype = np.dtype({"f32": np.float32, "f16": np.float16, "bf16": np.uint16, "i32": np.int32}[key])
+        flags |= KEYS[key] << 8
         self.flags = flags
         h = ctypes.c_void_p()
         _lib.check(_lib.load().radik_plan_create(ctypes.byref(h), _lib.i64_array(lens), _lib.i64_array(ks),
@@ -53,6 +59,8 @@ class Plan:
             raise _lib.RadiKError(_lib.RADIK_ESIZE, f"idx_in must hold {n_in} indices")
         if val_out.size < self.out_size or idx_out.size < self.out_size:
             raise _lib.RadiKError(_lib.RADIK_ESIZE, f"val_out/idx_out must hold {self.out_size} results")
+        if np.dtype(val_in.dtype) != self.val_dtype or np.dtype(val_out.dtype) != self.val_dtype:
+            raise _lib.RadiKError(_lib.RADIK_EINVAL, f"val_in/val_out must be {self.val_dtype} for key {self.key}")
         if np.dtype(idx_out.dtype).itemsize != self.idx_dtype.itemsize:
             raise _lib.RadiKError(_lib.RADIK_EINVAL, f"idx_out must be {self.idx_dtype}")
         if idx_in is not None and np.dtype(idx_in.dtype).itemsize != self.idx_dtype.itemsize:

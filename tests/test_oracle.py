@@ -427,3 +427,97 @@ def test_planted_mistakes_are_caught(plant, tmp_path):
                     f"-DORC_PLANT={plant}", "-o", out, os.path.join(ROOT, "oracle", "radik_oracle.c"), "-lm"],
                    check=True)
     assert _pins_pass(out) == (plant == 0)
+
+
+# ---------------------------------------------------------------- other key types
+def _numeric_topk(vals_f, lens, k, flags):
+    """numpy statement of the contract for data without NaN: order by numeric
+    value (numpy's own compare; -0 below +0), ties by position."""
+    out = []
+    off = 0
+    for n in lens:
+        v = np.asarray(vals_f[off:off + n])
+        neg0 = np.signbit(v) & (v == 0)
+        pos = np.arange(n)
+        # rank: largest first (descending value, +0 before -0) or smallest first
+        if flags & O.LARGEST:
+            sel = np.lexsort((pos, neg0, -v.astype(np.float64)))[:min(k, n)]
+        else:
+            sel = np.lexsort((pos, ~neg0, v.astype(np.float64)))[:min(k, n)]
+        sv = v[sel].astype(np.float64)
+        s0 = neg0[sel]
+        if flags & O.ASCEND:
+            order = sel[np.lexsort((sel, ~s0, sv))]
+        else:
+            order = sel[np.lexsort((sel, s0, -sv))]
+        out.append(order)
+        off += n
+    return out
+
+
+def _typed_cases(rng, kt):
+    n = int(rng.integers(1, 3000))
+    if kt == O.KEY_F16:
+        x = (rng.standard_normal(n) * 8).astype(np.float16)
+        x[rng.random(n) < 0.2] = np.float16(1.5)            # ties
+        x[rng.random(n) < 0.01] = np.array([np.inf, -np.inf, 0.0, -0.0, 6e-8], np.float16)[rng.integers(0, 5)]
+        return x, x.astype(np.float32)
+    if kt == O.KEY_BF16:
+        f = (rng.standard_normal(n) * 100).astype(np.float32)
+        f[rng.random(n) < 0.2] = np.float32(3.0)
+        b = O.f32_to_bf16(f)
+        b[rng.random(n) < 0.01] = np.array([0x7F80, 0xFF80, 0x0000, 0x8000, 0x0001], np.uint16)[rng.integers(0, 5)]
+        return b, O.bf16_to_f32(b)
+    x = rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    x[rng.random(n) < 0.2] = 7
+    x[:3] = [np.iinfo(np.int32).min, np.iinfo(np.int32).max, 0]
+    return x, x
+
+
+@pytest.mark.parametrize("kt", [1, 2, 3])
+def test_typed_exact_matches_numpy_order(kt):
+    rng = np.random.default_rng(100 + kt)
+    for _ in range(40):
+        x, xf = _typed_cases(rng, kt)
+        lens = [x.size // 3, x.size - x.size // 3]
+        k = int(rng.integers(1, 400))
+        for flags in (O.LARGEST | O.ASCEND, O.LARGEST, 0, O.ASCEND):
+            v, i = O.topk_exact_typed(kt, x, lens, k, flags)
+            exp = _numeric_topk(xf, lens, k, flags)
+            off = 0
+            for t, n in enumerate(lens):
+                m = len(exp[t])
+                assert np.array_equal(i[t, :m], exp[t] + 1)
+                assert np.array_equal(v[t, :m].view(np.uint16 if kt in (1, 2) else np.uint32),
+                                      x[off + exp[t]].view(np.uint16 if kt in (1, 2) else np.uint32))
+                assert np.all(i[t, m:] == -1)
+                off += n
+
+
+def test_typed_keys_closed_form():
+    # float16 / bfloat16: the key orders every pattern by IEEE totalOrder
+    h = np.arange(1 << 16, dtype=np.uint32)
+    k16 = np.array([O.key_typed(int(b), O.KEY_F16) for b in h[::97]], np.int64)
+    sign = h[::97] >> 15
+    tot = np.where(sign == 1, -(h[::97] & 0x7FFF).astype(np.int64) - 1, h[::97].astype(np.int64))
+    assert np.array_equal(np.argsort(k16, kind="stable"), np.argsort(tot, kind="stable"))
+    assert O.key_typed(0x8000, O.KEY_F16) < O.key_typed(0x0000, O.KEY_F16)  # -0 < +0
+    # int32: two's complement order
+    vals = np.array([-2**31, -5, -1, 0, 1, 2**31 - 1], np.int64)
+    ks = [O.key_typed(int(np.uint32(v & 0xFFFFFFFF)), O.KEY_I32) for v in vals]
+    assert ks == sorted(ks) and len(set(ks)) == len(ks)
+    # float32 through the typed entry equals the Float32 oracle
+    rng = np.random.default_rng(3)
+    x = _with_nans(rng, rng.standard_normal(5000))
+    for flags in (O.LARGEST | O.ASCEND, 0):
+        a = O.topk_exact(x, [2000, 3000], 150, flags)
+        b = O.topk_exact_typed(O.KEY_F32, x, [2000, 3000], 150, flags)
+        assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(a[1], b[1])
+
+
+def test_f32_to_bf16_rounding():
+    # round to nearest even on the dropped 16 bits
+    f = np.array([1.0, 1.00390625, 1.005859375, -2.5, 3.4e38, 1e-40], np.float32)
+    b = O.f32_to_bf16(f)
+    assert b[0] == 0x3F80 and b[1] == 0x3F80 and b[2] == 0x3F81  # ties to even, above half rounds up
+    assert O.bf16_to_f32(b)[3] == -2.5
