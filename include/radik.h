@@ -148,6 +148,24 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * LDS, small = short and medium tasks) and the number of executes timed. */
 int radik_plan_set_timing(radik_plan *plan, int enable);
 int radik_plan_timing(radik_plan *plan, double *ms8, int64_t *count);
+/* Temperature / top-p (nucleus) sampling from the sorted top-k of each task
+ * (no counterpart in the reference, whose path ends with the sorted [k, T]
+ * outputs, src/algorithm.jl:340-367): the step after the top-k in a serving
+ * loop over vocabulary logits.  Enqueued on `stream` after the plan's
+ * execute (no host synchronisation in between); reads the outputs that
+ * execute wrote (val_out / idx_out, descending: the plan must have
+ * largest = true and rev = true).  Per task, over its m = min(k, len)
+ * results v_1 >= v_2 >= ... (float32 arithmetic, one fixed summation order):
+ * w_i = exp((v_i - v_1) / temperature), c_i = w_1 + ... + w_i, the nucleus
+ * n = the first i with c_i >= top_p * c_m (m if none), and the sample r =
+ * the first i <= n with c_i > uniforms[t] * c_n.  tokens[t] = idx_out[r]
+ * (int64), ranks[t] = r - 1 (0-based); temperature <= 0 takes r = 1
+ * (greedy; uniforms unused); empty tasks give -1.  Device arrays:
+ * uniforms (num_tasks floats in [0, 1)), tokens, ranks.  Logits must not be
+ * NaN. */
+int radik_plan_sample(radik_plan *plan, const void *val_out, const void *idx_out, float temperature, float top_p,
+                      const float *uniforms, int64_t *tokens, int32_t *ranks, void *stream);
+
 /* How the last execute staged the plan's tasks (after the first execute):
  * info[0..8] = long tasks (the streaming passes), their elements (what
  * pass A reads), medium tasks, their elements, short tasks, their elements,

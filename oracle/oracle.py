@@ -259,3 +259,38 @@ def f32_to_bf16(x):
     nan = np.isnan(np.asarray(x, np.float32))
     out[nan] = ((u[nan] >> np.uint64(16)) | np.uint64(0x40)).astype(np.uint16)
     return out
+
+
+def sample_topp(vals_desc, m, temperature, top_p, u):
+    """Temperature / top-p sampling restated in float64 (the oracle of
+    radik_plan_sample): per row, over the first m descending values v,
+    w_i = exp((v_i - v_1) / T) (1 where v_i == v_1), c = cumsum(w), the
+    nucleus n = first i with c_i >= top_p * c_m (m if none), the sample r =
+    first i <= n with c_i > u * c_n.  Returns (rank r - 1, slack) per row;
+    slack is the float64 distance of the two decisions from their nearest
+    boundary, relative to c_m (a float32 evaluation may legitimately land on
+    the other side of a boundary closer than its rounding error)."""
+    vals_desc = np.asarray(vals_desc, np.float64)
+    ranks = np.full(len(m), -1, np.int64)
+    slack = np.full(len(m), np.inf)
+    for t, mt in enumerate(m):
+        if mt <= 0:
+            continue
+        v = vals_desc[t, :mt]
+        if not temperature > 0:
+            ranks[t] = 0
+            continue
+        w = np.where(v == v[0], 1.0, np.exp((v - v[0]) / np.float64(temperature)))
+        c = np.cumsum(w)
+        z = c[-1]
+        want = top_p * z
+        hit = np.nonzero(c >= want)[0]
+        n = hit[0] + 1 if hit.size else mt
+        cn = c[n - 1]
+        th = u[t] * cn
+        r = int(np.nonzero(c[:n] > th)[0][0]) + 1 if np.any(c[:n] > th) else n
+        ranks[t] = r - 1
+        d1 = np.min(np.abs(c - want)) / z
+        d2 = np.min(np.abs(c[:n] - th)) / z
+        slack[t] = min(d1, d2)
+    return ranks, slack

@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -19,6 +20,9 @@ const char* last_error();
 int diag_scan(const uint32_t* in, uint32_t* ow, uint32_t* ob, int64_t n, hipStream_t s);
 int diag_pass1(Pipeline& p, const DevFacts& f, uint32_t* hist_out, uint32_t* sel_out, hipStream_t s);
 int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
+int sample_topp(const void* vals, const void* idx, int idx_bytes, const int64_t* out_off, const int32_t* m,
+                int64_t num_tasks, int key_type, float temperature, float top_p, const float* u, int64_t* tokens,
+                int32_t* ranks, hipStream_t s);
 int noalloc_run(float* val_out, void* idx_out, int idx_bytes, const float* val_in, const void* idx_in,
                 const int64_t* task_lens, int64_t num_tasks, int64_t k, uint32_t tflags, hipStream_t s);
 }  // namespace radik
@@ -61,7 +65,12 @@ struct radik_plan {
   PhaseTimer timer;
   double ms[NUM_PHASES] = {};
   int64_t timed = 0;
+  // sampling (radik_plan_sample): per-task output offsets and result counts
+  int64_t* d_soff = nullptr;
+  int32_t* d_sm = nullptr;
   ~radik_plan() {
+    if (d_soff) (void)hipFree(d_soff);
+    if (d_sm) (void)hipFree(d_sm);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto& e : timer.ev)
@@ -506,6 +515,30 @@ int radik_plan_execute(radik_plan* p, void* val_out, void* idx_out, const void* 
   }
   RADIK_HIP(hipGraphLaunch(p->exec, s));
   return 0;
+}
+
+int radik_plan_sample(radik_plan* p, const void* val_out, const void* idx_out, float temperature, float top_p,
+                      const float* uniforms, int64_t* tokens, int32_t* ranks, void* stream) {
+  clear_error();
+  if (!p) return set_error(RADIK_EINVAL, "plan is NULL");
+  if (!(p->flags & RADIK_LARGEST) || (p->flags & RADIK_ASCEND))
+    return set_error(RADIK_EINVAL, "sampling needs a plan with largest = true and rev = true (descending results)");
+  const int64_t T = (int64_t)p->lens.size();
+  if (T == 0) return 0;
+  if (!val_out || !idx_out || !tokens || !ranks) return set_error(RADIK_EINVAL, "NULL buffer");
+  if (temperature > 0.0f && !uniforms) return set_error(RADIK_EINVAL, "uniforms is NULL");
+  if (!(top_p >= 0.0f)) return set_error(RADIK_EINVAL, "top_p must be >= 0");
+  if (!p->d_soff) {
+    std::vector<int32_t> m((size_t)T);
+    for (int64_t t = 0; t < T; ++t) m[(size_t)t] = (int32_t)std::min(p->lens[(size_t)t], p->ks[(size_t)t]);
+    if (hipMalloc((void**)&p->d_soff, (size_t)T * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc((void**)&p->d_sm, (size_t)T * sizeof(int32_t)) != hipSuccess)
+      return set_error(RADIK_ENOMEM, "allocation of the sampling tables failed");
+    RADIK_HIP(hipMemcpy(p->d_soff, p->out_off.data(), (size_t)T * sizeof(int64_t), hipMemcpyHostToDevice));
+    RADIK_HIP(hipMemcpy(p->d_sm, m.data(), (size_t)T * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  return sample_topp(val_out, idx_out, p->idx_bytes, p->d_soff, p->d_sm, T, key_type_of(p->flags), temperature,
+                     top_p, uniforms, tokens, ranks, (hipStream_t)stream);
 }
 
 int radik_plan_info(radik_plan* p, int64_t* info, int n) {

@@ -62,6 +62,7 @@ SIGNATURES = {
     "radik_plan_set_timing": (_INT, [_P, _INT]),
     "radik_plan_timing": (_INT, [_P, ctypes.POINTER(ctypes.c_double), _I64P]),
     "radik_plan_info": (_INT, [_P, _I64P, _INT]),
+    "radik_plan_sample": (_INT, [_P, _P, _P, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P]),
     "radik_plan_destroy": (_INT, [_P]),
     "radik_server_create": (_INT, [ctypes.POINTER(_P), _I64, _INT, _U32]),
     "radik_server_submit": (_INT, [_P, _P, _I64, _I64, _I64, _P, _P, _I64P]),

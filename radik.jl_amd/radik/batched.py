@@ -78,6 +78,24 @@ class Plan:
         _lib.check(_lib.load().radik_plan_set_options(self.handle, int(vload_bytes), int(blocks_per_cu),
                                                       int(small_cap), int(refine_cap), 1 if predict else 0))
 
+    def sample(self, val_out, idx_out, temperature, top_p, uniforms=None, tokens=None, ranks=None, stream=None):
+        """Temperature / top-p sampling from the results of the last execute
+        (radik_plan_sample; the plan must be largest=True, rev=True).
+        Returns (tokens int64[T], ranks int32[T]) device arrays."""
+        T = len(self.task_lens)
+        if tokens is None:
+            tokens = DeviceArray.zeros(T, np.int64)
+        if ranks is None:
+            ranks = DeviceArray.zeros(T, np.int32)
+        if uniforms is not None and uniforms.size < T:
+            raise _lib.RadiKError(_lib.RADIK_ESIZE, f"uniforms must hold {T} values")
+        if tokens.size < T or ranks.size < T:
+            raise _lib.RadiKError(_lib.RADIK_ESIZE, f"tokens/ranks must hold {T} entries")
+        _lib.check(_lib.load().radik_plan_sample(self.handle, val_out.ptr, idx_out.ptr, float(temperature),
+                                                 float(top_p), None if uniforms is None else uniforms.ptr,
+                                                 tokens.ptr, ranks.ptr, None if stream is None else stream.handle))
+        return tokens, ranks
+
     def info(self):
         """How the last execute staged the tasks (radik_plan_info)."""
         v = (ctypes.c_int64 * 9)()

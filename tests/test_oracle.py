@@ -521,3 +521,27 @@ def test_f32_to_bf16_rounding():
     b = O.f32_to_bf16(f)
     assert b[0] == 0x3F80 and b[1] == 0x3F80 and b[2] == 0x3F81  # ties to even, above half rounds up
     assert O.bf16_to_f32(b)[3] == -2.5
+
+
+# ---------------------------------------------------------------- sampling
+def test_sampling_oracle_closed_forms():
+    # weights 3, 2, 1 (T = 1): c = 3, 5, 6
+    v = np.log(np.array([[3.0, 2.0, 1.0]]))
+    for u, r in ((0.0, 0), (0.49, 0), (0.51, 1), (0.82, 1), (0.84, 2), (0.999, 2)):
+        assert O.sample_topp(v, [3], 1.0, 1.0, [u])[0][0] == r
+    # nucleus top_p = 0.5: c_i >= 3 at i = 1, only the first survives
+    assert O.sample_topp(v, [3], 1.0, 0.5, [0.99])[0][0] == 0
+    # top_p = 0.8: c_i >= 4.8 at i = 2; u * c_2 = 0.7 * 5 = 3.5 -> rank 1
+    assert O.sample_topp(v, [3], 1.0, 0.8, [0.7])[0][0] == 1
+    # greedy and equal values (uniform over the tie)
+    assert O.sample_topp(v, [3], 0.0, 0.9, [0.99])[0][0] == 0
+    eq = np.full((1, 8), 2.5)
+    for j in range(8):
+        assert O.sample_topp(eq, [8], 0.7, 1.0, [(j + 0.5) / 8])[0][0] == j
+    # temperature scales log-weights: T = 2 halves them (weights 3^.5, 2^.5, 1)
+    w = np.sqrt([3.0, 2.0, 1.0])
+    c = np.cumsum(w)
+    u = (c[0] + 0.01) / c[-1]
+    assert O.sample_topp(v, [3], 2.0, 1.0, [u])[0][0] == 1
+    # empty rows
+    assert O.sample_topp(np.zeros((1, 4)), [0], 1.0, 1.0, [0.3])[0][0] == -1
