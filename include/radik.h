@@ -139,6 +139,12 @@ int radik_plan_set_graph(radik_plan *plan, int use_graph);
  * suffices. */
 int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu, int small_cap, int refine_cap,
                            int predict);
+/* One tuning knob by name (results do not depend on them): "vload_bytes"
+ * (16/32/64), "blocks_per_cu", "small_cap", "refine_cap", "predict" (0/1)
+ * as in radik_plan_set_options, and "medium" (0/1: tasks of 2^11..2^16
+ * elements run whole in one block beside the streaming passes, or join
+ * the streaming passes and are re-read by the finishing kernel). */
+int radik_plan_set_option(radik_plan *plan, const char *name, int64_t value);
 /* Per-kernel timing: when enabled, execute runs eagerly (every kernel on the
  * plan's stream) and records HIP events between the phases; radik_plan_timing
  * returns the accumulated milliseconds of the 8 phases (sample, predict,

@@ -415,6 +415,40 @@ int radik_plan_set_options(radik_plan* p, int vload_bytes, int blocks_per_cu, in
   return 0;
 }
 
+int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
+  clear_error();
+  if (!p || !name) return set_error(RADIK_EINVAL, "plan or name is NULL");
+  const std::string n(name);
+  if (n == "medium") {
+    p->opts.medium = value != 0;
+
+  } else if (n == "predict") {
+    p->opts.predict = value != 0;
+  } else if (n == "blocks_per_cu") {
+    if (value < 1 || value > 64) return set_error(RADIK_EINVAL, "blocks_per_cu must be in 1..64");
+    p->opts.blocks_per_cu = (int)value;
+  } else if (n == "small_cap") {
+    if (value < 0 || value > INT32_MAX) return set_error(RADIK_EINVAL, "bad small_cap");
+    p->opts.small_cap = (uint32_t)value;
+  } else if (n == "refine_cap") {
+    if (value < 0 || value > INT32_MAX) return set_error(RADIK_EINVAL, "bad refine_cap");
+    p->opts.refine_cap = (uint32_t)value;
+  } else if (n == "vload_bytes") {
+    if (value != 16 && value != 32 && value != 64) return set_error(RADIK_EINVAL, "vload_bytes must be 16, 32 or 64");
+    const LaunchOpts keep = p->opts;
+    p->opts = opts_for(p->flags | RADIK_WITHPACKING, (int)value);
+    p->opts.blocks_per_cu = keep.blocks_per_cu;
+    p->opts.small_cap = keep.small_cap;
+    p->opts.refine_cap = keep.refine_cap;
+    p->opts.predict = keep.predict;
+    p->opts.medium = keep.medium;
+  } else {
+    return set_error(RADIK_EINVAL, "unknown plan option '%s'", name);
+  }
+  p->staged = false;  // re-stage (routing and task order may change) and re-capture
+  return 0;
+}
+
 int radik_plan_set_timing(radik_plan* p, int enable) {
   clear_error();
   if (!p) return set_error(RADIK_EINVAL, "plan is NULL");

@@ -369,47 +369,6 @@ __device__ __forceinline__ void sample_bounds(const uint64_t* v, uint32_t n, uin
   U = ((Ph + 1ull) << sh) - 1ull;  // (wraps to ~0 at the top)
 }
 
-// An upper bound U >= the q-th smallest (1 <= q <= n) of n distinct
-// composites v[0, n) in LDS that share the first digit `top`: the narrow
-// digits of the q-th are followed until a bin whose largest member it is; U
-// is the highest composite of that bin (sample_bounds' upper side alone).
-__device__ __forceinline__ uint64_t sample_upper(const uint64_t* v, uint32_t n, uint32_t q, uint32_t top,
-                                                 uint64_t vary, uint64_t same, uint32_t* lh8, uint32_t* wsum,
-                                                 uint32_t* res) {
-  constexpr int shifts[7] = {44, 36, 32, 24, 16, 8, 0};
-  constexpr int widths[7] = {8, 8, 4, 8, 8, 8, 8};
-  uint64_t P = top;
-  int sh = 52;
-  for (int d = 0; d < 7; ++d) {
-    const int s = shifts[d], w = widths[d];
-    const uint32_t nb = 1u << w;
-    const int hs = s + w;
-    if (((vary >> s) & (nb - 1)) == 0) {  // one bin
-      P = (P << w) | ((same >> s) & (nb - 1));
-      sh = s;
-      continue;
-    }
-    for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) lh8[i] = 0;
-    __syncthreads();
-    const uint64_t Pc = P;
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-      const uint64_t x = v[i];
-      if ((x >> hs) == Pc) hist_add(lh8, (uint32_t)(x >> s) & (nb - 1));
-    }
-    __syncthreads();
-    uint32_t c[1];
-    c[0] = (threadIdx.x < nb) ? lh8[threadIdx.x] : 0u;
-    block_find_bin<1>(c, q, wsum, res);
-    const uint32_t b = res[0], before = res[1], cnt = res[2];
-    __syncthreads();  // res and lh8 are rewritten next
-    P = (P << w) | b;
-    sh = s;
-    q -= before;
-    if (q == cnt) break;
-  }
-  return ((P + 1ull) << sh) - 1ull;  // (wraps to ~0 at the top)
-}
-
 // ---------------------------------------------------------------------------
 // Sampling: a predicted tasks owns G groups of SAMPLE_GROUP consecutive
 // elements spread evenly over it (group j starts at j * len / G).  Blocks take
@@ -417,16 +376,17 @@ __device__ __forceinline__ uint64_t sample_upper(const uint64_t* v, uint32_t n, 
 // the sample goes to shist.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ tasks,
-                                                  const int64_t* __restrict__ sbases, int ntasks,
-                                                  int64_t total_groups, uint32_t* __restrict__ shist,
-                                                  const uint32_t* __restrict__ bmap) {
-  __shared__ uint32_t lh[HIST_BINS];
+// Sample block b (of SAMPLE_GROUPS_PER_BLOCK groups; t: the task holding
+// its first group, from the host): adds the groups' first digits to their
+// tasks' sample histograms.
+__device__ __forceinline__ void sample_block(int64_t b, int t, const TaskDesc* __restrict__ tasks,
+                                             const int64_t* __restrict__ sbases, int ntasks, int64_t total_groups,
+                                             uint32_t* __restrict__ shist, uint32_t* lh) {
+  __shared__ int64_t gst[SAMPLE_GROUPS_PER_BLOCK];  // start of each group of the block in its task
   for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
-  int64_t g0 = (int64_t)blockIdx.x * SAMPLE_GROUPS_PER_BLOCK;
+  int64_t g0 = b * SAMPLE_GROUPS_PER_BLOCK;
   const int64_t g1 = min(g0 + (int64_t)SAMPLE_GROUPS_PER_BLOCK, total_groups);
   if (g0 >= g1) return;
-  int t = (int)bmap[blockIdx.x];  // = find_task(sbases, ntasks, g0), from the host
   __syncthreads();
   while (true) {
     const TaskDesc d = tasks[t];
@@ -436,19 +396,24 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
     const float sc = task_scaler(d);
     const bool scale = task_scales(d);
     const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
+    // group j of the task starts at j * len / G: one 64-bit division per
+    // group, by one thread each, instead of one per load
+    if (threadIdx.x < SAMPLE_GROUPS_PER_BLOCK) {
+      const int64_t g = g0 + threadIdx.x;
+      gst[threadIdx.x] = g < se ? (g - tb) * d.len / G : d.len;
+    }
+    __syncthreads();
     // all of this task's groups of the block in flight at once
     uint32_t xv[SAMPLE_GROUPS_PER_BLOCK];
 #pragma unroll
     for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
-      const int64_t g = g0 + u;
-      const int64_t i = (g - tb) * d.len / G + threadIdx.x;
-      xv[u] = (g < se && i < d.len) ? load_elem(d.in, i) : 0u;
+      const int64_t i = gst[u] + threadIdx.x;
+      xv[u] = (i < d.len) ? load_elem(d.in, i) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < SAMPLE_GROUPS_PER_BLOCK; ++u) {
-      const int64_t g = g0 + u;
-      const int64_t i = (g - tb) * d.len / G + threadIdx.x;
-      if (g < se && i < d.len) atomicAdd(&lh[rank_key(xv[u], sc, xm, scale) >> 20], 1u);
+      const int64_t i = gst[u] + threadIdx.x;
+      if (i < d.len) atomicAdd(&lh[rank_key(xv[u], sc, xm, scale) >> 20], 1u);
     }
     __syncthreads();
     flush_hist(lh, shist + (size_t)t * HIST_BINS);
@@ -460,15 +425,23 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
   }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ tasks,
+                                                  const int64_t* __restrict__ sbases, int ntasks,
+                                                  int64_t total_groups, uint32_t* __restrict__ shist,
+                                                  const uint32_t* __restrict__ bmap) {
+  __shared__ uint32_t lh[HIST_BINS];
+  sample_block(blockIdx.x, (int)bmap[blockIdx.x], tasks, sbases, ntasks, total_groups, shist, lh);
+}
+
 // ---------------------------------------------------------------------------
 // Prediction, one block per task; initialises the task's control block.
 // From the sample histogram it picks the first digit b_pred of the q-th
 // smallest sampled key, q = f*m + 4*sqrt(f*m) + 4 (f = sampled fraction,
 // m = min(k, len)); then, re-reading the sampled elements of that digit
 // (when the digit holds more than PRED_MIN and at most PRED_CAP of them, in
-// a sample of at most PRED_MAX_G groups),
-// it bounds the q-th smallest sampled composite (rank << 32 | position) from
-// above by (r_pred, p_pred), the position rounded up to the end of the gap
+// a sample of at most PRED_MAX_G groups) and ranking them by counting, it
+// takes the q-th smallest sampled composite (rank << 32 | position) as the
+// bound (r_pred, p_pred), the position rounded up to the end of the gap
 // after its sample group (the sample is contiguous groups, so positions are
 // only known per group).  Pass A spills exactly the composites at or below
 // that bound; the task holds at least m of them unless the sample is off by
@@ -479,18 +452,27 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
 // ---------------------------------------------------------------------------
 constexpr uint32_t PRED_MIN = 32;
 constexpr uint32_t PRED_CAP = 1024;
+static_assert(PRED_CAP == BLOCK * 4, "the prediction ranks PRED_CAP composites, 4 per thread");
 
-__global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ tasks,
-                                                   const int64_t* __restrict__ sbases,
-                                                   uint32_t* __restrict__ shist,
-                                                   TaskCtl* __restrict__ ctl) {
-  __shared__ uint32_t wsum[BLOCK / 64];
-  __shared__ uint32_t res[3];
-  __shared__ uint32_t lh8[256];
-  __shared__ uint32_t nsel;
-  __shared__ unsigned long long vb[2];
-  __shared__ uint64_t sv[PRED_CAP];
-  const int t = blockIdx.x;
+// LDS of a prediction beside its PRED_CAP-entry array sv.
+struct PredShared {
+  uint32_t wsum[BLOCK / 64];
+  uint32_t res[3];
+  uint32_t lh8[256];
+  uint32_t nsel;
+  unsigned long long vb[2];
+  uint64_t qv;
+  int64_t gst[PRED_MAX_G];  // sample group starts
+};
+
+__device__ __forceinline__ void predict_task(int t, const TaskDesc* __restrict__ tasks,
+                                             const int64_t* __restrict__ sbases, uint32_t* __restrict__ shist,
+                                             TaskCtl* __restrict__ ctl, PredShared& P, uint64_t* sv) {
+  uint32_t* wsum = P.wsum;
+  uint32_t* res = P.res;
+  uint32_t* lh8 = P.lh8;
+  uint32_t& nsel = P.nsel;
+  unsigned long long* vb = P.vb;
   const TaskDesc d = tasks[t];
   const int64_t G = sbases[t + 1] - sbases[t];
   const int64_t m = d.k < d.len ? (int64_t)d.k : d.len;
@@ -531,15 +513,17 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
       const float sc = task_scaler(d);
       const bool scale = task_scales(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-      uint64_t a = ~0ull, o = 0ull;
       constexpr int U = 16;
+      // group starts (one 64-bit division per group; G <= PRED_MAX_G)
+      if (threadIdx.x < (uint32_t)G) P.gst[threadIdx.x] = (int64_t)threadIdx.x * d.len / G;
+      __syncthreads();
       for (int64_t j0 = 0; j0 < G; j0 += U) {
         uint32_t xv[U];
         uint32_t ps[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int64_t i = (j0 + u) * d.len / G + threadIdx.x;
-          const bool ok = j0 + u < G && i < d.len;
+          const int64_t i = (j0 + u < G ? P.gst[j0 + u] : d.len) + threadIdx.x;
+          const bool ok = i < d.len;
           xv[u] = ok ? load_elem(d.in, i) : 0u;
           ps[u] = ok ? (uint32_t)i : 0xFFFFFFFFu;
         }
@@ -550,20 +534,34 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
           const uint64_t hm = ballot(hit);
           if (hm) {
             const uint32_t slot = wave_append(hit, hm, &nsel);
-            if (hit && slot < PRED_CAP) {
-              const uint64_t x = ((uint64_t)rk << 32) | ps[u];
-              sv[slot] = x;
-              a &= x;
-              o |= x;
-            }
+            if (hit && slot < PRED_CAP) sv[slot] = ((uint64_t)rk << 32) | ps[u];
           }
         }
       }
       __syncthreads();
-      block_and_or(a, o, vb);
       if (nsel == cnt && need > before && need - before <= cnt) {
-        const uint32_t qi = need - before;  // the q-th smallest is the qi-th of digit b
-        const uint64_t Ub = sample_upper(sv, cnt, qi, b, (uint64_t)(vb[0] ^ vb[1]), (uint64_t)vb[0], lh8, wsum, res);
+        // the q-th smallest sampled composite is the qi-th of digit b: sort
+        // the digit's sampled composites in registers (PRED_CAP = BLOCK * 4)
+        const uint32_t qi = need - before;
+        // the entry with exactly qi - 1 smaller ones (composites are
+        // distinct): each thread ranks its (up to) 4 entries against all
+        uint64_t x4[4];
+        uint32_t r4[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+          const uint32_t i = threadIdx.x + (uint32_t)l * BLOCK;
+          x4[l] = i < cnt ? sv[i] : ~0ull;
+        }
+        for (uint32_t j = 0; j < cnt; ++j) {
+          const uint64_t y = sv[j];
+#pragma unroll
+          for (int l = 0; l < 4; ++l) r4[l] += y < x4[l] ? 1u : 0u;
+        }
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+          if (threadIdx.x + (uint32_t)l * BLOCK < cnt && r4[l] == qi - 1) P.qv = x4[l];
+        __syncthreads();
+        const uint64_t Ub = P.qv;
         rp = (uint32_t)(Ub >> 32);
         pp = (uint32_t)Ub;
         if (pp != 0xFFFFFFFFu) {  // up to the position before the next group's start
@@ -585,6 +583,15 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ 
     o.p_pred = pp;
     ctl[t] = o;
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_predict(const TaskDesc* __restrict__ tasks,
+                                                   const int64_t* __restrict__ sbases,
+                                                   uint32_t* __restrict__ shist,
+                                                   TaskCtl* __restrict__ ctl) {
+  __shared__ PredShared P;
+  __shared__ uint64_t sv[PRED_CAP];
+  predict_task(blockIdx.x, tasks, sbases, shist, ctl, P, sv);
 }
 
 // ---------------------------------------------------------------------------

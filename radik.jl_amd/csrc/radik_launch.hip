@@ -84,16 +84,6 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
     RADIK_HIP(hipEventRecord(p.ev_join, p.side));
   }
   if (p.n_large) {
-    if (p.total_groups > 0) {
-      hipLaunchKernelGGL(k_sample, dim3(p.grid_s), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
-                         p.total_groups, p.d_shist, p.d_bmap + p.off_s);
-      RADIK_HIP(hipGetLastError());
-    }
-    if ((rc = mark())) return rc;
-    hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, p.d_shist,
-                       p.d_ctl);
-    RADIK_HIP(hipGetLastError());
-    if ((rc = mark())) return rc;
     // k_finish's LDS: cap entries for a task's winners + candidates (a power
     // of two >= 2048, which also holds the 4096-bin histogram of the wide
     // digits), plus the m results of fin tasks; results sorted in k_finish
@@ -119,10 +109,22 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
     const uint32_t fin_cap = sort_here ? cap : 0u;
     uint32_t emax = 1;
     while (sort_here && (uint32_t)BLOCK * emax < p.max_m) emax *= 2;
-    switch (o.unr) {
-      case 1: launch_stream_kernels_hist<1>(p, fin_cap, s); break;
-      case 4: launch_stream_kernels_hist<UNR_MAX>(p, fin_cap, s); break;
-      default: launch_stream_kernels_hist<2>(p, fin_cap, s); break;
+    {
+      if (p.total_groups > 0) {
+        hipLaunchKernelGGL(k_sample, dim3(p.grid_s), dim3(BLOCK), 0, s, p.d_large, p.d_sbases, (int)p.n_large,
+                           p.total_groups, p.d_shist, p.d_bmap + p.off_s);
+        RADIK_HIP(hipGetLastError());
+      }
+      if ((rc = mark())) return rc;
+      hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases,
+                         p.d_shist, p.d_ctl);
+      RADIK_HIP(hipGetLastError());
+      if ((rc = mark())) return rc;
+      switch (o.unr) {
+        case 1: launch_stream_kernels_hist<1>(p, fin_cap, s); break;
+        case 4: launch_stream_kernels_hist<UNR_MAX>(p, fin_cap, s); break;
+        default: launch_stream_kernels_hist<2>(p, fin_cap, s); break;
+      }
     }
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;

@@ -59,6 +59,7 @@ SIGNATURES = {
     "radik_plan_execute": (_INT, [_P, _P, _P, _P, _P, _P]),
     "radik_plan_set_graph": (_INT, [_P, _INT]),
     "radik_plan_set_options": (_INT, [_P, _INT, _INT, _INT, _INT, _INT]),
+    "radik_plan_set_option": (_INT, [_P, ctypes.c_char_p, _I64]),
     "radik_plan_set_timing": (_INT, [_P, _INT]),
     "radik_plan_timing": (_INT, [_P, ctypes.POINTER(ctypes.c_double), _I64P]),
     "radik_plan_info": (_INT, [_P, _I64P, _INT]),

@@ -104,6 +104,10 @@ class Plan:
                  "hist_chunks", "sample_groups", "has_big")
         return {nm: int(v[i]) for i, nm in enumerate(names)}
 
+    def set_option(self, name, value):
+        """One launch knob by name (radik_plan_set_option); results do not depend on it."""
+        _lib.check(_lib.load().radik_plan_set_option(self.handle, name.encode(), int(value)))
+
     def set_timing(self, on):
         _lib.check(_lib.load().radik_plan_set_timing(self.handle, 1 if on else 0))
 
