@@ -304,9 +304,10 @@ def spiky_rows(rng, rows, cols):
 def serving(radik, seconds=3.0, seed=5150, pool_size=12, budget=16 << 30, inflight=4):
     """Poisson arrivals of rows x 128256 requests (rows U{1..256}, k in {1, 50,
     1024}), served (a) by one synchronous drop-in call per request in arrival
-    order and (b) by the continuous-batching server, at the same frozen rate:
-    lambda = 0.7 / (mean plain service time).  Latency = arrival -> results
-    complete; p50/p99 and rows/s for both."""
+    order and (b) by the continuous-batching server, at the same frozen rates:
+    lambda = 0.7 / (mean plain service time) and a saturating 1.5 / (mean
+    plain service time).  Latency = arrival -> results complete; p50/p99 and
+    rows/s for both forms at both loads."""
     from radik import DeviceArray
     cols = 128256
     rng = np.random.default_rng(seed)
@@ -330,48 +331,52 @@ def serving(radik, seconds=3.0, seed=5150, pool_size=12, budget=16 << 30, inflig
         for req in pool:
             plain(req)
     s_mean = (time.perf_counter() - t0) / (reps * len(pool))
-    lam = 0.7 / s_mean
-    n_req = max(50, int(seconds * lam))
-    arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / lam, n_req))
-    which = np.random.default_rng(seed + 2).integers(0, len(pool), n_req)
-    rows_total = int(sum(pool[w][0] for w in which))
 
     def wait_until(t):
         while time.perf_counter() < t:
             pass
 
-    lat_plain = np.empty(n_req)
-    t0 = time.perf_counter()
-    for i in range(n_req):
-        wait_until(t0 + arr[i])
-        plain(pool[which[i]])
-        lat_plain[i] = time.perf_counter() - (t0 + arr[i])
-    span_plain = time.perf_counter() - t0
-
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)
     srv = radik.Server(budget_bytes=budget, max_inflight=inflight, largest=True, rev=True)
     for req in pool:  # warm the slots' buffers
         srv.wait(srv.submit(req[2], req[0], cols, req[1], req[3], req[4]))
-    tickets = np.empty(n_req, dtype=np.int64)
-    lag = np.empty(n_req)
-    t0 = time.perf_counter()
-    for i in range(n_req):
-        wait_until(t0 + arr[i])
-        lag[i] = time.perf_counter() - (t0 + arr[i])  # submit delay behind the arrival
-        rows, k, x, vo, io = pool[which[i]]
-        tickets[i] = srv.submit(x, rows, cols, k, vo, io)
-    lat_srv = np.array([srv.wait(int(t)) for t in tickets]) * 1e-3 + lag
-    span_srv = time.perf_counter() - t0
-    st = srv.stats()
+    out = {"plain_service_ms": round(s_mean * 1e3, 4), "budget_bytes": budget, "slots": inflight}
+    for name, load in (("load70", 0.7), ("load150", 1.5)):
+        lam = load / s_mean
+        n_req = max(50, int(seconds * lam))
+        arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / lam, n_req))
+        which = np.random.default_rng(seed + 2).integers(0, len(pool), n_req)
+        rows_total = int(sum(pool[w][0] for w in which))
+        lat_plain = np.empty(n_req)
+        t0 = time.perf_counter()
+        for i in range(n_req):
+            wait_until(t0 + arr[i])
+            plain(pool[which[i]])
+            lat_plain[i] = time.perf_counter() - (t0 + arr[i])
+        span_plain = time.perf_counter() - t0
+        st0 = srv.stats()
+        tickets = np.empty(n_req, dtype=np.int64)
+        lag = np.empty(n_req)
+        t0 = time.perf_counter()
+        for i in range(n_req):
+            wait_until(t0 + arr[i])
+            lag[i] = time.perf_counter() - (t0 + arr[i])  # submit delay behind the arrival
+            rows, k, x, vo, io = pool[which[i]]
+            tickets[i] = srv.submit(x, rows, cols, k, vo, io)
+        lat_srv = np.array([srv.wait(int(t)) for t in tickets]) * 1e-3 + lag
+        span_srv = time.perf_counter() - t0
+        st = srv.stats()
+        out[name] = {
+            "lambda_req_s": round(lam, 1), "requests": n_req,
+            "plain": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
+                      "rows_s": round(rows_total / span_plain, 1)},
+            "served": {"p50_ms": pct(lat_srv, 50), "p99_ms": pct(lat_srv, 99),
+                       "rows_s": round(rows_total / span_srv, 1), "launches": st["launches"] - st0["launches"],
+                       "max_coalesced": st["max_coalesced"]},
+        }
+        log(name, out[name])
     srv.close()
-    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)
-    return {
-        "lambda_req_s": round(lam, 1), "requests": n_req, "plain_service_ms": round(s_mean * 1e3, 4),
-        "plain": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
-                  "rows_s": round(rows_total / span_plain, 1)},
-        "served": {"p50_ms": pct(lat_srv, 50), "p99_ms": pct(lat_srv, 99),
-                   "rows_s": round(rows_total / span_srv, 1), "launches": st["launches"],
-                   "max_coalesced": st["max_coalesced"]},
-    }
+    return out
 
 
 def main():
@@ -384,6 +389,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--only-batched", action="store_true",
                     help="time the batched plan only (profiling runs)")
+    ap.add_argument("--only-serving", action="store_true", help="run the serving comparison only")
     ap.add_argument("--workload", default="uneven", choices=("uneven", "single"),
                     help="batched-plan workload: the uneven batch, or one 2^28 task (profiling runs)")
     ap.add_argument("--k", type=int, default=256)
@@ -416,6 +422,9 @@ def main():
         log(f"copy kernel: 5 x {nb} bytes")
     if args.tune:
         tune_all(radik, args.tune)
+        return
+    if args.only_serving:
+        print(json.dumps(serving(radik)), flush=True)
         return
     t0 = time.perf_counter()
     if args.workload == "single":

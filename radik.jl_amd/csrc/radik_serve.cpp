@@ -1,12 +1,19 @@
 // radik_serve.cpp -- serving engine: continuous batching of top-k requests.
 //
 // submit() queues a request (a rows x cols row-major matrix on the device,
-// one task per row, its own k).  A dispatcher thread coalesces everything
-// pending into one launch whenever an in-flight slot is free and the batch
-// fits the slot's share of the device budget; each slot owns a stream and a
-// pipeline (descriptors, histograms, spill buffer), so up to max_inflight
-// batches overlap on the GPU.  A completer thread stamps each batch's
-// completion on the host clock when its event fires.
+// one task per row, its own k).  A dispatcher thread coalesces pending
+// requests, oldest first, into one launch whenever an in-flight slot is
+// free and the device budget has room: the budget is one pool shared by all
+// slots, charged with each in-flight batch's workspace footprint.  A batch
+// takes requests up to the slot share (budget / slots), so a fresh arrival
+// waits behind at most one bounded batch per slot; a request larger than
+// the share runs alone once the pool has room for it (when the device is
+// idle, up to the whole budget).  Each slot owns a stream and a pipeline
+// (descriptors, histograms, spill buffers, kept between batches up to the
+// share: a slot that ran a larger batch frees them afterwards), so up to
+// `slots` batches overlap on the GPU.  One waiter thread per slot blocks on
+// the slot's completion event and stamps its requests' latency (host submit
+// -> completion) the moment it fires.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -41,16 +48,21 @@ struct Request {
 struct Result {
   bool done = false;
   double latency_ms = 0;
+  Clock::time_point t_submit;
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  Pipeline pipe;
+  std::unique_ptr<Pipeline> pipe;
   std::vector<TaskSpec> specs;
   std::vector<int64_t> tickets;
+  int64_t bytes = 0;      // footprint charged to the pool while in flight
+  int64_t rows = 0;
   bool busy = false;      // taken by the dispatcher
   bool launched = false;  // its batch and completion event are enqueued
+  std::condition_variable cv;  // waiter: batch launched, or stop
+  std::thread waiter;
 };
 
 }  // namespace
@@ -58,19 +70,21 @@ struct Slot {
 struct radik_server {
   int device = 0;
   int64_t budget = 0;
-  int64_t slot_budget = 0;
+  int64_t share = 0;   // batch bound: budget / slots
   uint32_t flags = 0;
   std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
-  std::condition_variable cv_work;   // dispatcher: new request or free slot
-  std::condition_variable cv_done;   // waiters: a result completed
+  std::condition_variable cv_work;   // dispatcher: new request, freed slot or budget
+  std::condition_variable cv_done;   // callers of wait/drain: a result completed
   std::deque<Request> pending;
   std::unordered_map<int64_t, Result> results;
   int64_t next_ticket = 1;
-  bool stop = false;
+  int64_t pool_used = 0;  // footprint of the batches in flight
+  bool stop = false;          // no new requests; the dispatcher ends once pending is empty
+  bool stop_waiters = false;  // the dispatcher has ended
   int inflight = 0;
-  std::thread dispatcher, completer;
-  std::atomic<int64_t> launches{0}, rows_done{0}, max_coalesced{0};
+  std::thread dispatcher;
+  std::atomic<int64_t> launches{0}, rows_done{0}, max_coalesced{0}, max_pool{0};
   int error = 0;
   std::string error_msg;
 };
@@ -78,6 +92,12 @@ struct radik_server {
 static int64_t footprint(int64_t rows, int64_t cols) {
   // spill buffer + histogram + descriptors per row (upper bound)
   return rows * (cols * 8 + HIST_BINS * 4 + 2 * (int64_t)sizeof(TaskDesc) + (int64_t)sizeof(TaskCtl) + 8);
+}
+
+static void raise_max(std::atomic<int64_t>& a, int64_t v) {
+  int64_t prev = a.load();
+  while (v > prev && !a.compare_exchange_weak(prev, v)) {
+  }
 }
 
 static void dispatch_loop(radik_server* s) {
@@ -89,36 +109,40 @@ static void dispatch_loop(radik_server* s) {
     s->error_msg = "device query failed in dispatcher";
     return;
   }
+  const uint32_t tflags = s->flags & 0x0Fu & ~RADIK_WITHIDXIN;
+  const int kt = (int)((s->flags & RADIK_KEY_MASK) >> RADIK_KEY_SHIFT);
+  const int eb = key_bytes(kt);
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     // (after stop, requests still pending are dispatched as slots free up)
     s->cv_work.wait(lk, [&] {
-      return (s->stop && s->pending.empty()) || (!s->pending.empty() && s->inflight < (int)s->slots.size());
+      if (s->pending.empty()) return s->stop;
+      return s->inflight < (int)s->slots.size() && s->pool_used + s->pending.front().bytes <= s->budget;
     });
     if (s->pending.empty()) return;  // stopping, nothing left
     Slot* slot = nullptr;
     for (auto& sl : s->slots)
       if (!sl->busy) { slot = sl.get(); break; }
-    if (!slot) continue;
-    // Coalesce everything pending that fits the slot budget (FIFO).
+    // Coalesce, oldest first, up to the slot share (a larger first request
+    // runs alone) and the pool's room.
     std::vector<Request> batch;
     int64_t used = 0;
     while (!s->pending.empty()) {
       const Request& r = s->pending.front();
-      if (!batch.empty() && used + r.bytes > s->slot_budget) break;
+      if (!batch.empty() && (used + r.bytes > s->share || s->pool_used + used + r.bytes > s->budget)) break;
       used += r.bytes;
       batch.push_back(r);
       s->pending.pop_front();
     }
     slot->busy = true;
     slot->launched = false;
+    slot->bytes = used;
+    s->pool_used += used;
     s->inflight++;
+    raise_max(s->max_pool, s->pool_used);
     lk.unlock();
     slot->specs.clear();
     slot->tickets.clear();
-    const uint32_t tflags = s->flags & 0x0Fu & ~RADIK_WITHIDXIN;
-    const int kt = (int)((s->flags & RADIK_KEY_MASK) >> RADIK_KEY_SHIFT);
-    const int eb = key_bytes(kt);
     int64_t nrows = 0;
     for (const Request& r : batch) {
       for (int64_t i = 0; i < r.rows; ++i) {
@@ -135,65 +159,59 @@ static void dispatch_loop(radik_server* s) {
       nrows += r.rows;
       slot->tickets.push_back(r.ticket);
     }
+    slot->rows = nrows;
+    if (!slot->pipe) slot->pipe.reset(new Pipeline());
     LaunchOpts o;
     o.key_type = kt;
-    int rc = stage_tasks(slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, o, slot->stream);
-    if (!rc) rc = upload(slot->pipe, slot->stream);
-    if (!rc) rc = enqueue(slot->pipe, *f, o, slot->stream, nullptr);
-    if (!rc && hipEventRecord(slot->done, slot->stream) != hipSuccess) rc = RADIK_EHIP;
+    int rc = stage_tasks(*slot->pipe, slot->specs.data(), (int64_t)slot->specs.size(), *f, o, slot->stream);
+    if (!rc) rc = upload(*slot->pipe, slot->stream);
+    if (!rc) rc = enqueue(*slot->pipe, *f, o, slot->stream, nullptr);
+    if (hipEventRecord(slot->done, slot->stream) != hipSuccess && !rc) rc = RADIK_EHIP;
     s->launches++;
-    s->rows_done += nrows;
-    int64_t prev = s->max_coalesced.load();
-    while ((int64_t)batch.size() > prev && !s->max_coalesced.compare_exchange_weak(prev, (int64_t)batch.size())) {
-    }
-    if (rc) (void)hipEventRecord(slot->done, slot->stream);  // released once the stream drains
+    raise_max(s->max_coalesced, (int64_t)batch.size());
     lk.lock();
     if (rc) {
       s->error = rc;
       s->error_msg = "batch launch failed";
     }
     slot->launched = true;
-    s->cv_work.notify_all();
+    slot->cv.notify_all();
   }
 }
 
-static void complete_loop(radik_server* s) {
+// One per slot: waits for the slot's batch to complete, stamps its requests
+// and returns the slot and its footprint to the pool.
+static void waiter_loop(radik_server* s, Slot* slot) {
   (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
-    bool any = false;
-    {
-      std::unique_lock<std::mutex> lk(s->mu);
-      if (s->stop && s->inflight == 0 && s->pending.empty()) return;
-      for (auto& sl : s->slots) {
-        if (!sl->busy) continue;
-        any = true;
-        if (!sl->launched || hipEventQuery(sl->done) != hipSuccess) continue;
-        const auto now = Clock::now();
-        for (int64_t t : sl->tickets) {
-          auto it = s->results.find(t);
-          if (it == s->results.end()) continue;
-          it->second.done = true;
-        }
-        sl->busy = false;
-        sl->launched = false;
-        s->inflight--;
-        // latency: stamped against the submit time kept in the request map
-        for (int64_t t : sl->tickets) {
-          auto it = s->results.find(t);
-          if (it != s->results.end())
-            it->second.latency_ms = std::chrono::duration<double, std::milli>(now.time_since_epoch()).count() -
-                                    it->second.latency_ms;  // latency_ms held the submit stamp
-        }
-        s->cv_work.notify_all();
-        s->cv_done.notify_all();
-      }
+    slot->cv.wait(lk, [&] { return slot->launched || s->stop_waiters; });
+    if (!slot->launched) return;  // stopping, no batch left
+    lk.unlock();
+    const hipError_t e = hipEventSynchronize(slot->done);
+    const auto now = Clock::now();
+    // A slot that ran a batch above the share frees its buffers, so the
+    // slots together never hold more than the budget between batches.
+    if (slot->bytes > s->share) slot->pipe.reset();
+    lk.lock();
+    if (e != hipSuccess && !s->error) {
+      s->error = RADIK_EHIP;
+      s->error_msg = "a batch failed on the device";
     }
-    if (!any) {
-      std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
-    } else {
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    for (int64_t t : slot->tickets) {
+      auto it = s->results.find(t);
+      if (it == s->results.end()) continue;
+      it->second.done = true;
+      it->second.latency_ms = std::chrono::duration<double, std::milli>(now - it->second.t_submit).count();
     }
+    s->rows_done += slot->rows;
+    s->pool_used -= slot->bytes;
+    slot->bytes = 0;
+    slot->busy = false;
+    slot->launched = false;
+    s->inflight--;
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
   }
 }
 
@@ -214,7 +232,7 @@ int radik_server_create(radik_server** out, int64_t budget_bytes, int max_inflig
   int rc = dev_facts(&f);
   if (rc) return rc;
   s->budget = budget_bytes;
-  s->slot_budget = budget_bytes / max_inflight;
+  s->share = budget_bytes / max_inflight;
   s->flags = flags;
   for (int i = 0; i < max_inflight; ++i) {
     std::unique_ptr<Slot> sl(new Slot());
@@ -223,8 +241,8 @@ int radik_server_create(radik_server** out, int64_t budget_bytes, int max_inflig
     s->slots.push_back(std::move(sl));
   }
   radik_server* raw = s.get();
+  for (auto& sl : raw->slots) sl->waiter = std::thread(waiter_loop, raw, sl.get());
   raw->dispatcher = std::thread(dispatch_loop, raw);
-  raw->completer = std::thread(complete_loop, raw);
   *out = s.release();
   return 0;
 }
@@ -238,9 +256,9 @@ int radik_server_submit(radik_server* s, const void* in, int64_t rows, int64_t c
                      (long long)cols, (long long)k);
   if (!in || (k > 0 && (!val_out || !idx_out))) return set_error(RADIK_EINVAL, "NULL buffer");
   const int64_t bytes = footprint(rows, cols);
-  if (bytes > s->slot_budget)
-    return set_error(RADIK_EADMIT, "request needs %lld bytes of workspace, a batch may use %lld", (long long)bytes,
-                     (long long)s->slot_budget);
+  if (bytes > s->budget)
+    return set_error(RADIK_EADMIT, "request needs %lld bytes of workspace, the budget is %lld", (long long)bytes,
+                     (long long)s->budget);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return set_error(RADIK_EINVAL, "server is stopping");
   Request r;
@@ -254,7 +272,7 @@ int radik_server_submit(radik_server* s, const void* in, int64_t rows, int64_t c
   r.bytes = bytes;
   r.t_submit = Clock::now();
   Result res;
-  res.latency_ms = std::chrono::duration<double, std::milli>(r.t_submit.time_since_epoch()).count();
+  res.t_submit = r.t_submit;
   s->results[r.ticket] = res;
   s->pending.push_back(r);
   *ticket = r.ticket;
@@ -315,8 +333,15 @@ int radik_server_destroy(radik_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  // the dispatcher launches what is still pending as the waiters free slots
   if (s->dispatcher.joinable()) s->dispatcher.join();
-  if (s->completer.joinable()) s->completer.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop_waiters = true;
+    for (auto& sl : s->slots) sl->cv.notify_all();
+  }
+  for (auto& sl : s->slots)
+    if (sl->waiter.joinable()) sl->waiter.join();
   for (auto& sl : s->slots) {
     if (sl->stream) (void)hipStreamSynchronize(sl->stream);
     if (sl->done) (void)hipEventDestroy(sl->done);

@@ -103,3 +103,53 @@ def test_plan_per_task_k_matches_separate_calls():
         assert np.array_equal(i[oo:oo + k], ei[0])
         off += n
         oo += k
+
+
+def test_server_large_request_runs_alone_within_the_pool():
+    # budget 1 GiB over 4 slots: a request above the slot share (256 MiB)
+    # but within the budget is admitted and runs in a batch of its own
+    rng = np.random.default_rng(606)
+    rows, cols, k = 160, 200000, 64
+    x = spiky(rng, rows, cols)
+    d = DeviceArray.from_host(x)
+    vo = DeviceArray.zeros((k, rows), np.float32)
+    io = DeviceArray.zeros((k, rows), np.int32)
+    small = []
+    for _ in range(6):
+        xs = spiky(rng, 2, 5000)  # (the device copies stay referenced until the results are read)
+        small.append((xs, DeviceArray.from_host(xs), DeviceArray.zeros((10, 2), np.float32),
+                      DeviceArray.zeros((10, 2), np.int32)))
+    srv = radik.Server(budget_bytes=1 << 30, max_inflight=4, largest=True, rev=True)
+    say("large request beside small ones")
+    tickets = [srv.submit(ds, 2, 5000, 10, v, i) for xs, ds, v, i in small[:3]]
+    tickets.append(srv.submit(d, rows, cols, k, vo, io))
+    tickets += [srv.submit(ds, 2, 5000, 10, v, i) for xs, ds, v, i in small[3:]]
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+    ev, ei = O.topk_exact(x, [cols] * rows, k, O.LARGEST)
+    assert np.array_equal(vo.to_host().view(np.uint32), ev.T.view(np.uint32))
+    assert np.array_equal(io.to_host(), ei.T)
+    for xs, ds, v, i in small:
+        ev, ei = O.topk_exact(xs, [5000] * 2, 10, O.LARGEST)
+        assert np.array_equal(v.to_host().view(np.uint32), ev.T.view(np.uint32)) and np.array_equal(i.to_host(), ei.T)
+
+
+def test_server_close_with_a_backlog():
+    # more requests than slots, closed without waiting: close drains them
+    rng = np.random.default_rng(707)
+    srv = radik.Server(budget_bytes=256 << 20, max_inflight=1, largest=True, rev=True)
+    reqs = []
+    for _ in range(24):
+        x = spiky(rng, 3, 20000)
+        reqs.append((x, DeviceArray.from_host(x), DeviceArray.zeros((5, 3), np.float32),
+                     DeviceArray.zeros((5, 3), np.int32)))
+    say("backlog of", len(reqs))
+    for x, d, vo, io in reqs:
+        srv.submit(d, 3, 20000, 5, vo, io)
+    srv.close()
+    radik.synchronize()
+    for x, d, vo, io in reqs:
+        ev, ei = O.topk_exact(x, [20000] * 3, 5, O.LARGEST)
+        assert np.array_equal(vo.to_host().view(np.uint32), ev.T.view(np.uint32))
+        assert np.array_equal(io.to_host(), ei.T)
