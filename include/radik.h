@@ -211,6 +211,20 @@ int radik_diag_pass1(const float *val_in, const int64_t *task_lens, int64_t num_
  * (counterpart of @warpreduce, src/utils.jl:84-100): out_wave[i] = inclusive
  * scan within i's wave, out_block[i] = inclusive scan within i's block of 256. */
 int radik_diag_scan(const uint32_t *in, uint32_t *out_wave, uint32_t *out_block, int64_t n, void *stream);
+/* Host logic without a device call (for CPU tests): pass A's chunk table
+ * over `total` elements (nchunks chunks, starts[0..nchunks] when cap is
+ * enough), the routing of tasks as a launch stages them (cls: 0 short,
+ * 1 medium, 2 long with a sample of groups[t] groups, 3 long without), the
+ * serving engine's next batch (how many of the pending footprints it takes
+ * given the slot share, the pool in use and the budget) and a request's
+ * footprint. */
+int radik_diag_plan_chunks(int64_t total, int64_t blocks, int64_t spread, int guided, int64_t gran,
+                           int64_t min_chunk, int64_t *starts, int64_t cap, int64_t *nchunks);
+int radik_diag_route(const int64_t *task_lens, const int64_t *k_per_task, int64_t num_tasks, int small_cap,
+                     int predict, int medium, int32_t *cls, int64_t *groups);
+int radik_diag_batch(const int64_t *bytes, int64_t n, int64_t share, int64_t pool_used, int64_t budget,
+                     int64_t *take);
+int radik_diag_footprint(int64_t rows, int64_t cols, int64_t *bytes);
 /* A plain 16-byte streaming copy of `bytes` (multiple of 16): the known-size
  * reference that scales the profiler's HBM traffic counters. */
 int radik_diag_copy(void *dst, const void *src, int64_t bytes, void *stream);

@@ -23,6 +23,10 @@ int diag_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 int sample_topp(const void* vals, const void* idx, int idx_bytes, const int64_t* out_off, const int32_t* m,
                 int64_t num_tasks, int key_type, float temperature, float top_p, const float* u, int64_t* tokens,
                 int32_t* ranks, hipStream_t s);
+int64_t diag_plan_chunks(int64_t total, int64_t blocks, int64_t spread, int guided, int64_t gran, int64_t min_chunk,
+                         int64_t* out, int64_t cap);
+void diag_route(const int64_t* lens, const int64_t* ks, int64_t n, uint32_t small_cap, int predict, int medium,
+                int32_t* cls, int64_t* groups);
 int noalloc_run(float* val_out, void* idx_out, int idx_bytes, const float* val_in, const void* idx_in,
                 const int64_t* task_lens, int64_t num_tasks, int64_t k, uint32_t tflags, hipStream_t s);
 }  // namespace radik
@@ -646,6 +650,25 @@ int radik_diag_pass1(const float* val_in, const int64_t* task_lens, int64_t num_
   rc = diag_pass1(pipe, all, hist_out, sel_out, s);
   if (rc) return rc;
   RADIK_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int radik_diag_plan_chunks(int64_t total, int64_t blocks, int64_t spread, int guided, int64_t gran,
+                           int64_t min_chunk, int64_t* starts, int64_t cap, int64_t* nchunks) {
+  clear_error();
+  if (!nchunks || total < 0 || blocks < 1 || spread < 1 || gran < 1 || min_chunk < 1)
+    return set_error(RADIK_EINVAL, "bad arguments");
+  *nchunks = diag_plan_chunks(total, blocks, spread, guided, gran, min_chunk, starts, cap);
+  if (starts && cap < *nchunks + 1) return set_error(RADIK_ESIZE, "starts needs %lld entries", (long long)(*nchunks + 1));
+  return 0;
+}
+
+int radik_diag_route(const int64_t* task_lens, const int64_t* k_per_task, int64_t num_tasks, int small_cap,
+                     int predict, int medium, int32_t* cls, int64_t* groups) {
+  clear_error();
+  if (num_tasks < 0 || (num_tasks > 0 && (!task_lens || !k_per_task || !cls || !groups)) || small_cap < 0)
+    return set_error(RADIK_EINVAL, "bad arguments");
+  diag_route(task_lens, k_per_task, num_tasks, (uint32_t)small_cap, predict, medium, cls, groups);
   return 0;
 }
 

@@ -476,6 +476,34 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   return 0;
 }
 
+// Host-logic diagnostics (no device call): the chunk table and the routing
+// of tasks, as stage_tasks computes them.
+int64_t diag_plan_chunks(int64_t total, int64_t blocks, int64_t spread, int guided, int64_t gran, int64_t min_chunk,
+                         int64_t* out, int64_t cap) {
+  const int64_t n = plan_chunks(total, blocks, spread, guided != 0, gran, min_chunk, nullptr);
+  if (out && cap >= n + 1) plan_chunks(total, blocks, spread, guided != 0, gran, min_chunk, out);
+  return n;
+}
+
+// cls[t]: 0 short (k_small), 1 medium (k_medium), 2 long and sampled, 3 long
+// without a sample; groups[t]: sample groups.
+void diag_route(const int64_t* lens, const int64_t* ks, int64_t n, uint32_t small_cap, int predict, int medium,
+                int32_t* cls, int64_t* groups) {
+  LaunchOpts o;
+  o.predict = predict != 0;
+  o.medium = medium != 0;
+  for (int64_t t = 0; t < n; ++t) {
+    const int64_t g = o.predict ? sample_groups(lens[t], ks[t]) : 0;
+    groups[t] = 0;
+    if (is_short(lens[t], small_cap)) cls[t] = 0;
+    else if (is_medium(lens[t], ks[t], g > 0, o)) cls[t] = 1;
+    else {
+      cls[t] = g > 0 ? 2 : 3;
+      groups[t] = g;
+    }
+  }
+}
+
 int upload(Pipeline& p, hipStream_t s) {
   // one copy of every host-built array (the arena's prefix)
   if (p.upload_bytes)

@@ -94,6 +94,17 @@ static int64_t footprint(int64_t rows, int64_t cols) {
   return rows * (cols * 8 + HIST_BINS * 4 + 2 * (int64_t)sizeof(TaskDesc) + (int64_t)sizeof(TaskCtl) + 8);
 }
 
+// How many pending requests (footprints bytes[0..n), oldest first) the next
+// batch takes: 0 when the oldest does not fit the pool's room (budget -
+// pool_used); else it and the following ones while the batch stays within
+// the slot share and the room (a first request above the share goes alone).
+static int64_t batch_take(const int64_t* bytes, int64_t n, int64_t share, int64_t pool_used, int64_t budget) {
+  if (n <= 0 || pool_used + bytes[0] > budget) return 0;
+  int64_t used = bytes[0], i = 1;
+  while (i < n && used + bytes[i] <= share && pool_used + used + bytes[i] <= budget) used += bytes[i++];
+  return i;
+}
+
 static void raise_max(std::atomic<int64_t>& a, int64_t v) {
   int64_t prev = a.load();
   while (v > prev && !a.compare_exchange_weak(prev, v)) {
@@ -112,6 +123,7 @@ static void dispatch_loop(radik_server* s) {
   const uint32_t tflags = s->flags & 0x0Fu & ~RADIK_WITHIDXIN;
   const int kt = (int)((s->flags & RADIK_KEY_MASK) >> RADIK_KEY_SHIFT);
   const int eb = key_bytes(kt);
+  std::vector<int64_t> footprints;
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     // (after stop, requests still pending are dispatched as slots free up)
@@ -125,13 +137,14 @@ static void dispatch_loop(radik_server* s) {
       if (!sl->busy) { slot = sl.get(); break; }
     // Coalesce, oldest first, up to the slot share (a larger first request
     // runs alone) and the pool's room.
+    footprints.clear();
+    for (const Request& r : s->pending) footprints.push_back(r.bytes);
+    const int64_t take = batch_take(footprints.data(), (int64_t)footprints.size(), s->share, s->pool_used, s->budget);
     std::vector<Request> batch;
     int64_t used = 0;
-    while (!s->pending.empty()) {
-      const Request& r = s->pending.front();
-      if (!batch.empty() && (used + r.bytes > s->share || s->pool_used + used + r.bytes > s->budget)) break;
-      used += r.bytes;
-      batch.push_back(r);
+    for (int64_t i = 0; i < take; ++i) {
+      used += s->pending.front().bytes;
+      batch.push_back(s->pending.front());
       s->pending.pop_front();
     }
     slot->busy = true;
@@ -313,6 +326,21 @@ int radik_server_drain(radik_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   s->cv_done.wait(lk, [&] { return s->error != 0 || (s->pending.empty() && s->inflight == 0); });
   if (s->error) return set_error(s->error, "%s", s->error_msg.c_str());
+  return 0;
+}
+
+int radik_diag_batch(const int64_t* bytes, int64_t n, int64_t share, int64_t pool_used, int64_t budget,
+                     int64_t* take) {
+  clear_error();
+  if (!take || n < 0 || (n > 0 && !bytes)) return set_error(RADIK_EINVAL, "bad arguments");
+  *take = batch_take(bytes, n, share, pool_used, budget);
+  return 0;
+}
+
+int radik_diag_footprint(int64_t rows, int64_t cols, int64_t* bytes) {
+  clear_error();
+  if (!bytes || rows < 0 || cols < 0) return set_error(RADIK_EINVAL, "bad arguments");
+  *bytes = footprint(rows, cols);
   return 0;
 }
 

@@ -75,6 +75,10 @@ SIGNATURES = {
     "radik_diag_pass1": (_INT, [_P, _I64P, _I64, _I64, _U32, _P, _P, _P]),
     "radik_diag_scan": (_INT, [_P, _P, _P, _I64, _P]),
     "radik_diag_copy": (_INT, [_P, _P, _I64, _P]),
+    "radik_diag_plan_chunks": (_INT, [_I64, _I64, _I64, _INT, _I64, _I64, _I64P, _I64, _I64P]),
+    "radik_diag_route": (_INT, [_I64P, _I64P, _I64, _INT, _INT, _INT, ctypes.POINTER(ctypes.c_int32), _I64P]),
+    "radik_diag_batch": (_INT, [_I64P, _I64, _I64, _I64, _I64, _I64P]),
+    "radik_diag_footprint": (_INT, [_I64, _I64, _I64P]),
     "radik_device_count": (_INT, [ctypes.POINTER(_INT)]),
     "radik_set_device": (_INT, [_INT]),
     "radik_device_info": (_INT, [ctypes.POINTER(_INT), _I64P, _I64P, ctypes.c_char_p, _INT]),
