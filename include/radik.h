@@ -33,13 +33,17 @@ extern "C" {
  * then hold that type; pass them cast to the declared pointer type.
  * Ordering: the float types by their IEEE-754 total order (-NaN < -Inf <
  * ... < -0 < +0 < ... < +Inf < +NaN), Int32 by value.  WITHSCALE takes
- * Float32 keys only. */
+ * Float32 keys only.  Float64 keys (bits 8-10 = 4) run through the
+ * workspace path (radik_topk_radix_select) and radik_topk_typed only, with
+ * k <= RADIK_F64_MAX_K; plans and the serving engine reject them. */
 #define RADIK_KEY_SHIFT 8
 #define RADIK_KEY_F32  0x000u /* Float32 (default)              */
 #define RADIK_KEY_F16  0x100u /* Float16 (IEEE binary16)         */
 #define RADIK_KEY_BF16 0x200u /* BFloat16                        */
 #define RADIK_KEY_I32  0x300u /* Int32                           */
-#define RADIK_KEY_MASK 0x300u
+#define RADIK_KEY_F64  0x400u /* Float64                         */
+#define RADIK_KEY_MASK 0x700u
+#define RADIK_F64_MAX_K 8192
 
 /* Status codes. */
 #define RADIK_OK            0
@@ -76,7 +80,10 @@ int radik_ws_destroy(radik_ws *ws);
  * task_lens: HOST array of num_tasks lengths; val_out/idx_out: device [k,
  * num_tasks] column-major.  Outputs per task: min(k, len) sorted results,
  * then 0.0f / -1 padding.  Synchronises the stream before returning, like
- * the reference (algorithm.jl:353). */
+ * the reference (algorithm.jl:353).  Each task holds fewer than 2^31
+ * elements with Int32 indices and fewer than 2^32 with Int64 (positions are
+ * 32-bit inside the selection keys; the reference's Int64 counters,
+ * algorithm.jl:150-154, reach further): longer tasks give RADIK_EINVAL. */
 int radik_topk_radix_select(radik_ws *ws, float *val_out, void *idx_out, int idx_bytes,
                             const float *val_in, const void *idx_in, const int64_t *task_lens,
                             int64_t num_tasks, int64_t k, uint32_t flags, int vload_bytes, void *stream);
@@ -91,9 +98,10 @@ int radik_topk(float *val_out, void *idx_out, int idx_bytes, int64_t out_rows, i
                int64_t num_tasks, int64_t k, int largest, int rev, void *stream);
 
 /* topk! for any key type: key_type 0 Float32, 1 Float16, 2 BFloat16,
- * 3 Int32 (RADIK_KEY_* >> RADIK_KEY_SHIFT); val_in / val_out hold that
- * type.  Otherwise radik_topk (src/algorithm.jl:477-522 generalised past
- * the reference's Float32-only check, algorithm.jl:492-494). */
+ * 3 Int32, 4 Float64 (RADIK_KEY_* >> RADIK_KEY_SHIFT); val_in / val_out
+ * hold that type.  Otherwise radik_topk (src/algorithm.jl:477-522
+ * generalised past the reference's Float32-only check,
+ * algorithm.jl:492-494). */
 int radik_topk_typed(int key_type, void *val_out, void *idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
                      const void *val_in, const void *indices, int64_t n_indices, const int64_t *task_lens,
                      int64_t num_tasks, int64_t k, int largest, int rev, void *stream);

@@ -294,3 +294,47 @@ def sample_topp(vals_desc, m, temperature, top_p, u):
         d2 = np.min(np.abs(c[:n] - th)) / z
         slack[t] = min(d1, d2)
     return ranks, slack
+
+
+# Float64 keys: restated with numpy (the C restatement works on 32-bit keys).
+KEY_F64 = 4
+KEY_DTYPES[KEY_F64] = np.float64
+
+
+def keys64(x):
+    """Order-preserving 64-bit key of float64 values: the key transform of
+    RadiK.jl src/utils.jl:29-33 widened to 64 bits (bits ^ (sign ? ~0 :
+    1 << 63)); unsigned order of the keys is IEEE totalOrder."""
+    b = np.ascontiguousarray(x, np.float64).view(np.uint64)
+    return np.where(b >> np.uint64(63) != 0, ~b, b ^ np.uint64(1 << 63))
+
+
+def topk_exact_f64(val_in, task_lens, k, flags, idx_in=None, idx_dtype=np.int32):
+    """[T, k] float64 values and indices by the contract's direct order:
+    per task the m = min(k, n) smallest (rank, position), rank = ~key for
+    LARGEST else key; written sorted by (ASCEND ? key : ~key, position)
+    (the final sort, src/algorithm.jl:340-352); indices idx_in[pos] or
+    pos + 1; slots past m hold 0.0 / -1 (algorithm.jl:355-365)."""
+    x = np.ascontiguousarray(val_in, np.float64)
+    lens = [int(n) for n in task_lens]
+    T = len(lens)
+    vo = np.zeros((T, k), np.float64)
+    io = np.full((T, k), -1, idx_dtype)
+    ones = np.uint64(0xFFFFFFFFFFFFFFFF)
+    off = 0
+    for t, n in enumerate(lens):
+        seg = x[off:off + n]
+        key = keys64(seg)
+        rank = key ^ ones if flags & LARGEST else key
+        pos = np.arange(n, dtype=np.int64)
+        m = min(k, n)
+        sel = np.lexsort((pos, rank))[:m]
+        okey = key[sel] if flags & ASCEND else key[sel] ^ ones
+        sel = sel[np.lexsort((sel, okey))]
+        vo[t, :m] = seg[sel]
+        if idx_in is not None:
+            io[t, :m] = np.asarray(idx_in)[off + sel]
+        else:
+            io[t, :m] = sel + 1
+        off += n
+    return vo, io

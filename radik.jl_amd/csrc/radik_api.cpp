@@ -42,6 +42,10 @@ struct radik_ws {
   std::vector<TaskSpec> specs;
   hipStream_t last_stream = nullptr;  // the stream of the last call (its work may still reference the buffers)
   bool used = false;
+  F64Pipe* f64 = nullptr;  // buffers of Float64 calls (made on the first one)
+  ~radik_ws() {
+    if (f64) f64_destroy(f64);
+  }
 };
 
 struct radik_plan {
@@ -97,8 +101,9 @@ static inline int key_type_of(uint32_t flags) { return (int)((flags & RADIK_KEY_
 
 // WITHSCALE subtracts in Float32 (utils.jl:62-76); other key types reject it.
 static int check_key_flags(uint32_t flags) {
-  if ((flags & RADIK_WITHSCALE) && key_type_of(flags) != KEY_F32)
-    return set_error(RADIK_EINVAL, "WITHSCALE needs Float32 keys");
+  const int kt = key_type_of(flags);
+  if (kt >= NUM_KEY_TYPES && kt != KEY_F64) return set_error(RADIK_EINVAL, "unknown key type %d", kt);
+  if ((flags & RADIK_WITHSCALE) && kt != KEY_F32) return set_error(RADIK_EINVAL, "WITHSCALE needs Float32 keys");
   return 0;
 }
 
@@ -241,7 +246,16 @@ static int ws_select(radik_ws* ws, void* val_out, void* idx_out, int idx_bytes, 
   }
   ws->last_stream = s;
   ws->used = true;
-  rc = run_tasks(ws->pipe, ws->specs, opts_for(flags, vload_bytes), s, nullptr);
+  if (key_type_of(flags) == KEY_F64) {
+    if (k > RADIK_F64_MAX_K)
+      return set_error(RADIK_EINVAL, "Float64 keys take k <= %d, got %lld", RADIK_F64_MAX_K, (long long)k);
+    DevFacts* f = nullptr;
+    if ((rc = dev_facts(&f))) return rc;
+    if ((rc = f64_stage(&ws->f64, ws->specs.data(), num_tasks, s))) return rc;
+    rc = f64_enqueue(ws->f64, s);
+  } else {
+    rc = run_tasks(ws->pipe, ws->specs, opts_for(flags, vload_bytes), s, nullptr);
+  }
   if (rc) return rc;
   RADIK_HIP(hipStreamSynchronize(s));
   return 0;
@@ -258,7 +272,8 @@ int radik_topk_radix_select(radik_ws* ws, float* val_out, void* idx_out, int idx
 static int topk_impl(int key_type, void* val_out, void* idx_out, int idx_bytes, int64_t out_rows, int64_t out_cols,
                      const void* val_in, const void* indices, int64_t n_indices, const int64_t* task_lens,
                      int64_t num_tasks, int64_t k, int largest, int rev, void* stream) {
-  if (key_type < 0 || key_type >= NUM_KEY_TYPES) return set_error(RADIK_EINVAL, "unknown key type %d", key_type);
+  if (key_type < 0 || (key_type >= NUM_KEY_TYPES && key_type != KEY_F64))
+    return set_error(RADIK_EINVAL, "unknown key type %d", key_type);
   if (num_tasks < 0 || (num_tasks > 0 && !task_lens)) return set_error(RADIK_EINVAL, "bad task_lens");
   if (out_rows != k || out_cols != num_tasks)
     return set_error(RADIK_ESIZE, "val_out must have size (%lld, %lld), got (%lld, %lld)", (long long)k,
@@ -369,6 +384,8 @@ int radik_plan_create(radik_plan** plan, const int64_t* task_lens, const int64_t
   if (flags & ~kFlagMask) return set_error(RADIK_EINVAL, "unknown flag bits 0x%x", flags & ~kFlagMask);
   int rc = check_key_flags(flags);
   if (rc) return rc;
+  if (key_type_of(flags) == KEY_F64)
+    return set_error(RADIK_EINVAL, "Float64 keys run through radik_topk_typed and the workspace path, not plans");
   int64_t total = 0, maxlen = 0;
   rc = check_lens(task_lens, num_tasks, idx_bytes, &total, &maxlen);
   if (rc) return rc;

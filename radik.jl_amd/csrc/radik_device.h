@@ -21,8 +21,11 @@ constexpr int KEY_F32 = 0;   // float32 (the reference's only type)
 constexpr int KEY_F16 = 1;   // IEEE binary16
 constexpr int KEY_BF16 = 2;  // bfloat16
 constexpr int KEY_I32 = 3;   // int32
-constexpr int NUM_KEY_TYPES = 4;
-inline int key_bytes(int kt) { return (kt == KEY_F16 || kt == KEY_BF16) ? 2 : 4; }
+constexpr int NUM_KEY_TYPES = 4;  // key types of the 32-bit-key path
+// Float64 keys take a path of their own (radik_f64.hip): their key needs all
+// 64 bits of a composite.
+constexpr int KEY_F64 = 4;
+inline int key_bytes(int kt) { return (kt == KEY_F16 || kt == KEY_BF16) ? 2 : (kt == KEY_F64 ? 8 : 4); }
 
 constexpr uint32_t NO_BIN = 0xFFFFFFFFu;
 constexpr int HIST_BINS = 4096;  // first digit: rank-key bits [31:20]
@@ -114,6 +117,16 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) {
   return __builtin_amdgcn_ballot_w64(p);
+}
+
+// Wave-aggregated append: every lane with `p` gets a distinct slot from
+// *ctr; one atomic per wave.  Must be reached by all active lanes together.
+__device__ __forceinline__ uint32_t wave_append(bool p, uint64_t mask, uint32_t* ctr) {
+  const uint64_t act = ballot(true);
+  uint32_t base = 0;
+  if (lanes_below(act) == 0) base = atomicAdd(ctr, (uint32_t)__builtin_popcountll(mask));
+  base = __builtin_amdgcn_readfirstlane(base);
+  return base + lanes_below(mask);
 }
 
 // Inclusive prefix sum across the 64 lanes of a wave with DPP row shifts and

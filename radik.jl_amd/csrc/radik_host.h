@@ -179,4 +179,13 @@ struct PhaseTimer {
 int upload(Pipeline& p, hipStream_t s);
 int enqueue(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipStream_t s, PhaseTimer* tm);
 
+// The Float64 path (radik_f64.hip): its own grow-only buffers per workspace;
+// staging copies the descriptors synchronously, enqueue launches the pass
+// sequence on the stream.  f64_setup opts its sort into the LDS it needs.
+struct F64Pipe;
+int f64_stage(F64Pipe** p, const TaskSpec* specs, int64_t n, hipStream_t s);
+int f64_enqueue(F64Pipe* p, hipStream_t s);
+void f64_destroy(F64Pipe* p);
+int f64_setup(size_t lds_budget);
+
 }  // namespace radik

@@ -205,16 +205,6 @@ __device__ __forceinline__ void put_result(const TaskDesc& d, uint32_t slot, uin
   else glb(reinterpret_cast<int32_t*>(d.idx_out))[slot] = (int32_t)pos;
 }
 
-// Wave-aggregated append: every lane with `p` gets a distinct slot from
-// *ctr; one atomic per wave.  Must be reached by all active lanes together.
-__device__ __forceinline__ uint32_t wave_append(bool p, uint64_t mask, uint32_t* ctr) {
-  const uint64_t act = ballot(true);
-  uint32_t base = 0;
-  if (lanes_below(act) == 0) base = atomicAdd(ctr, (uint32_t)__builtin_popcountll(mask));
-  base = __builtin_amdgcn_readfirstlane(base);
-  return base + lanes_below(mask);
-}
-
 // Per-wave LDS staging of appended entries: lanes with `p` write their entry
 // into the wave's stage; when the stage could overflow on the next append
 // (or at the end of a segment) the wave reserves space for all of them with
@@ -458,7 +448,6 @@ static_assert(PRED_CAP == BLOCK * 4, "the prediction ranks PRED_CAP composites, 
 struct PredShared {
   uint32_t wsum[BLOCK / 64];
   uint32_t res[3];
-  uint32_t lh8[256];
   uint32_t nsel;
   unsigned long long vb[2];
   uint64_t qv;
@@ -470,7 +459,6 @@ __device__ __forceinline__ void predict_task(int t, const TaskDesc* __restrict__
                                              TaskCtl* __restrict__ ctl, PredShared& P, uint64_t* sv) {
   uint32_t* wsum = P.wsum;
   uint32_t* res = P.res;
-  uint32_t* lh8 = P.lh8;
   uint32_t& nsel = P.nsel;
   unsigned long long* vb = P.vb;
   const TaskDesc d = tasks[t];

@@ -61,7 +61,8 @@ int dev_facts(DevFacts** out) {
     size_t budget = (size_t)(lds > 0 ? lds : 65536);
     int occ[NUM_KEY_TYPES] = {4, 4, 4, 4};
     const bool ok = launch_setup_kf32(budget, &occ[KEY_F32]) == 0 && launch_setup_kf16(budget, &occ[KEY_F16]) == 0 &&
-                    launch_setup_kbf16(budget, &occ[KEY_BF16]) == 0 && launch_setup_ki32(budget, &occ[KEY_I32]) == 0;
+                    launch_setup_kbf16(budget, &occ[KEY_BF16]) == 0 && launch_setup_ki32(budget, &occ[KEY_I32]) == 0 &&
+                    f64_setup(budget) == 0;
     if (!ok) {
       (void)hipGetLastError();
       budget = 65536;

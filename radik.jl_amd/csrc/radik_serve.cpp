@@ -238,6 +238,8 @@ int radik_server_create(radik_server** out, int64_t budget_bytes, int max_inflig
     return set_error(RADIK_EINVAL, "budget must be > 0 and max_inflight in 1..16");
   if ((flags & RADIK_WITHSCALE) && (flags & RADIK_KEY_MASK))
     return set_error(RADIK_EINVAL, "WITHSCALE needs Float32 keys");
+  if ((int)((flags & RADIK_KEY_MASK) >> RADIK_KEY_SHIFT) >= NUM_KEY_TYPES)
+    return set_error(RADIK_EINVAL, "the serving engine takes Float32, Float16, BFloat16 or Int32 keys");
   std::unique_ptr<radik_server> s(new (std::nothrow) radik_server());
   if (!s) return set_error(RADIK_ENOMEM, "out of host memory");
   RADIK_HIP(hipGetDevice(&s->device));

@@ -14,7 +14,8 @@ RADIK_ASCEND = 0x02
 RADIK_WITHSCALE = 0x04
 RADIK_WITHIDXIN = 0x08
 RADIK_WITHPACKING = 0x10
-RADIK_KEY_F32, RADIK_KEY_F16, RADIK_KEY_BF16, RADIK_KEY_I32 = 0x000, 0x100, 0x200, 0x300
+RADIK_KEY_F32, RADIK_KEY_F16, RADIK_KEY_BF16, RADIK_KEY_I32, RADIK_KEY_F64 = 0x000, 0x100, 0x200, 0x300, 0x400
+RADIK_F64_MAX_K = 8192
 
 RADIK_OK = 0
 RADIK_EINVAL = 1

@@ -64,13 +64,15 @@ def _flags(LARGEST, ASCEND, WITHSCALE, WITHIDXIN, WITHPACKING):
     return f
 
 
-KEYS = {"f32": 0, "f16": 1, "bf16": 2, "i32": 3}
-_KEY_OF_DTYPE = {np.dtype(np.float32): "f32", np.dtype(np.float16): "f16", np.dtype(np.int32): "i32"}
+KEYS = {"f32": 0, "f16": 1, "bf16": 2, "i32": 3, "f64": 4}
+_KEY_OF_DTYPE = {np.dtype(np.float32): "f32", np.dtype(np.float16): "f16", np.dtype(np.int32): "i32",
+                 np.dtype(np.float64): "f64"}
 
 
 def key_type(data, key=None):
-    """Key type code of `data`: float32, float16 and int32 arrays by their
-    dtype; bfloat16 data are uint16 bit patterns with key="bf16".  The
+    """Key type code of `data`: float32, float16, int32 and float64 arrays
+    by their dtype; bfloat16 data are uint16 bit patterns with key="bf16"
+    (float64 keys: topk / topk_ / topk_radix_select_ only, k <= 8192).  The
     reference takes Float32 only (algorithm.jl:492-494) and raises for
     anything else; here the other types run the same path on their own
     order-preserving keys."""
@@ -86,7 +88,7 @@ def key_type(data, key=None):
     if k is None:
         # algorithm.jl:492-494 (the reference raises MethodError earlier, Q10)
         raise RadiKError(_lib.RADIK_EINVAL,
-                         f"RadiK supports Float32, Float16, BFloat16 (uint16 bits, key='bf16') and Int32 "
+                         f"RadiK supports Float32, Float16, BFloat16 (uint16 bits, key='bf16'), Int32 and Float64 "
                          f"input data. Got {data.dtype}.")
     return KEYS[k]
 

@@ -32,7 +32,8 @@ class Plan:
         if key not in KEYS:
             raise _lib.RadiKError(_lib.RADIK_EINVAL, f"unknown key type {key!r}")
         self.key = key
-        self.val_dtype = np.dtype({"f32": np.float32, "f16": np.float16, "bf16": np.uint16, "i32": np.int32}[key])
+        self.val_dtype = np.dtype({"f32": np.float32, "f16": np.float16, "bf16": np.uint16, "i32": np.int32,
+                                   "f64": np.float64}[key])
         flags |= KEYS[key] << 8
         self.flags = flags
         h = ctypes.c_void_p()

@@ -1,10 +1,13 @@
-"""Key types beyond the reference's Float32 (Float16, BFloat16, Int32),
-through the C-ABI, against the oracle's typed restatement: bit-exact value
-patterns and indices (ties to the lowest position).  Short, medium and long
-(sampled, and over 2^20 elements) tasks, ties and special values (+-0,
-+-Inf, NaN, the integer extremes), both selection directions and output
-orders, the drop-in and the batched plan, and a result set beyond 4096
-(the multi-block sort)."""
+"""Key types beyond the reference's Float32 (Float16, BFloat16, Int32,
+Float64), through the C-ABI, against the oracle's typed restatements:
+bit-exact value patterns and indices (ties to the lowest position).  Short,
+medium and long (sampled, and over 2^20 elements) tasks, ties and special
+values (+-0, +-Inf, NaN, subnormals, the integer extremes), both selection
+directions and output orders, the drop-in and the batched plan, and a result
+set beyond 4096 (the multi-block sort).  Float64 runs its own path (drop-in
+and workspace): exact ties across the threshold, custom Int32 and Int64
+indices, k up to its limit of 8192, and the rejections (k above it, plans,
+WITHSCALE)."""
 import sys
 
 import numpy as np
@@ -112,4 +115,80 @@ def test_keytype_rejects_withscale_and_bad_dtypes():
     with pytest.raises(radik.RadiKError):
         radik.topk_radix_select_(vo, io, ws, d, None, [100], 10, WITHSCALE=True)
     with pytest.raises(radik.RadiKError):
-        radik.topk(DeviceArray.from_host(np.zeros(100, np.float64)), 10)
+        radik.topk(DeviceArray.from_host(np.zeros(100, np.int64)), 10)
+
+
+# ------------------------------------------------------------------ Float64
+def make_f64(n, rng, levels=0):
+    if levels:
+        x = rng.integers(0, levels, n).astype(np.float64) * 0.125 - 3.0
+    else:
+        x = rng.standard_normal(n) * 1e3
+        x[rng.random(n) < 0.03] = 2718.25  # ties near the top
+    sp = rng.integers(0, n, max(1, n // 5000))
+    x[sp] = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 5e-324, -5e-324])[rng.integers(0, 8, sp.size)]
+    return x
+
+
+def check64(got_v, got_i, ev, ei, name):
+    bad = np.nonzero(np.asarray(got_v).view(np.uint64) != np.asarray(ev).view(np.uint64))
+    assert bad[0].size == 0, f"{name}: {bad[0].size} value mismatches, first at {tuple(b[0] for b in bad)}"
+    bad = np.nonzero(got_i.astype(np.int64) != ei.astype(np.int64))
+    assert bad[0].size == 0, f"{name}: {bad[0].size} index mismatches, first at {tuple(b[0] for b in bad)}"
+
+
+def test_f64_batch_drop_in():
+    rng = np.random.default_rng(500)
+    lens = [700, 2048, 5000, 40000, 65536, 200000, 1 << 20, 0, 1500]
+    x = make_f64(sum(lens), rng)
+    d = DeviceArray.from_host(x)
+    for k in (1, 100, 1000):
+        for largest, rev in ((True, False), (False, True), (True, True), (False, False)):
+            flags = (O.LARGEST if largest else 0) | (0 if rev else O.ASCEND)
+            say("f64 drop-in", k, largest, rev)
+            ev, ei = O.topk_exact_f64(x, lens, k, flags)
+            v, i = radik.topk(d, lens, k, largest=largest, rev=rev)
+            check64(v.to_host(), i.to_host(), ev.T, ei.T, f"f64 topk k{k} l{largest} r{rev}")
+
+
+def test_f64_large_task_ties_indices_and_max_k():
+    rng = np.random.default_rng(501)
+    n = 1 << 22
+    idx = rng.permutation(n).astype(np.int32)
+    di = DeviceArray.from_host(idx)
+    ws = radik.RadiKWorkspace(n, 1)
+    for levels in (0, 64):  # 64 levels: exact ties across the threshold
+        x = make_f64(n, rng, levels)
+        d = DeviceArray.from_host(x)
+        for k in (64, 5000, O_MAX_K):
+            for flags in (O.LARGEST | O.ASCEND, 0):
+                say("f64 large", levels, k, flags)
+                ev, ei = O.topk_exact_f64(x, [n], k, flags, idx_in=idx)
+                vo = DeviceArray.zeros((k, 1), np.float64)
+                io = DeviceArray.zeros((k, 1), np.int32)
+                radik.topk_radix_select_(vo, io, ws, d, di, [n], k, LARGEST=bool(flags & O.LARGEST),
+                                         ASCEND=bool(flags & O.ASCEND), WITHIDXIN=True)
+                check64(vo.to_host(), io.to_host(), ev.T, ei.T, f"f64 large L{levels} k{k} f{flags}")
+
+
+O_MAX_K = 8192
+
+
+def test_f64_limits():
+    x = DeviceArray.from_host(np.zeros(10000, np.float64))
+    with pytest.raises(radik.RadiKError):
+        radik.topk(x, O_MAX_K + 1)
+    with pytest.raises(radik.RadiKError):
+        radik.Plan([10000], 10, key="f64")
+    ws = radik.RadiKWorkspace(10000, 1)
+    vo = DeviceArray.zeros((10, 1), np.float64)
+    io = DeviceArray.zeros((10, 1), np.int32)
+    with pytest.raises(radik.RadiKError):
+        radik.topk_radix_select_(vo, io, ws, x, None, [10000], 10, WITHSCALE=True)
+    # Int64 indices through the drop-in
+    rng = np.random.default_rng(502)
+    y = make_f64(30000, rng)
+    ii = rng.permutation(30000).astype(np.int64)
+    ev, ei = O.topk_exact_f64(y, [30000], 500, O.LARGEST, idx_in=ii, idx_dtype=np.int64)
+    v, i = radik.topk(DeviceArray.from_host(y), 500, indices=DeviceArray.from_host(ii), rev=True)
+    check64(v.to_host(), i.to_host(), ev[0], ei[0], "f64 int64 indices")

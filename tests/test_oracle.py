@@ -515,6 +515,51 @@ def test_typed_keys_closed_form():
         assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(a[1], b[1])
 
 
+def test_f64_keys_are_ieee_total_order():
+    # totalOrder as a signed integer: negative patterns -(magnitude bits) - 1
+    rng = np.random.default_rng(64)
+    b = rng.integers(0, 2**63, 4000, dtype=np.int64).astype(np.uint64)
+    b[::2] |= np.uint64(1 << 63)
+    b[:8] = np.array([0, 1 << 63, 0x7FF0000000000000, 0xFFF0000000000000, 0x7FF8000000000000,
+                      0xFFF8000000000001, 1, (1 << 63) | 1], np.uint64)
+    mag = (b & np.uint64(0x7FFFFFFFFFFFFFFF)).astype(np.int64)
+    tot = np.where(b >> np.uint64(63) != 0, -mag - 1, mag)
+    k = O.keys64(b.view(np.float64))
+    assert np.array_equal(np.argsort(k, kind="stable"), np.argsort(tot, kind="stable"))
+    assert O.keys64(np.array([-0.0]))[0] < O.keys64(np.array([0.0]))[0]
+
+
+def test_f64_exact_matches_the_float32_oracle_on_widened_data():
+    # float32 -> float64 is exact and keeps the order of every non-NaN
+    # value and of quiet NaNs, so the C restatement on the float32 data
+    # fixes the float64 restatement's selection, ties and output order
+    rng = np.random.default_rng(65)
+    for rep in range(30):
+        n = int(rng.integers(1, 4000))
+        x = (rng.standard_normal(n) * 10).astype(np.float32)
+        x[rng.random(n) < 0.2] = np.float32(2.5)
+        sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 1e-45, -1e-45], np.float32)
+        x[rng.random(n) < 0.02] = sp[rng.integers(0, sp.size)]
+        lens = [n // 3, n - n // 3]
+        k = int(rng.integers(1, 500))
+        idx = rng.permutation(n).astype(np.int32) if rep % 2 else None
+        for flags in (O.LARGEST | O.ASCEND, O.LARGEST, 0, O.ASCEND):
+            ev, ei = O.topk_exact(x, lens, k, flags, idx_in=idx)
+            gv, gi = O.topk_exact_f64(x.astype(np.float64), lens, k, flags, idx_in=idx)
+            assert np.array_equal(gi, ei)
+            assert np.array_equal(gv.astype(np.float32).view(np.uint32), ev.view(np.uint32))
+
+
+def test_f64_exact_matches_numpy_sort_on_distinct_values():
+    rng = np.random.default_rng(66)
+    x = rng.standard_normal(10000) * 1e200
+    v, i = O.topk_exact_f64(x, [10000], 300, O.LARGEST)
+    assert np.array_equal(v[0], np.sort(x)[::-1][:300])
+    assert np.array_equal(i[0], np.argsort(-x)[:300] + 1)
+    v, i = O.topk_exact_f64(x, [10000], 300, O.ASCEND)
+    assert np.array_equal(v[0], np.sort(x)[:300])
+
+
 def test_f32_to_bf16_rounding():
     # round to nearest even on the dropped 16 bits
     f = np.array([1.0, 1.00390625, 1.005859375, -2.5, 3.4e38, 1e-40], np.float32)
