@@ -41,6 +41,14 @@ constexpr int SAMPLE_GROUPS_PER_BLOCK = 16;
 // SORT_MAX_RUNS of them per task.
 constexpr uint32_t SORT_CHUNK = BLOCK * 8;
 constexpr uint32_t SORT_MAX_RUNS = 64;
+// Rank tail of tasks over 2^20 elements with a covering spill: their listed
+// winners and candidates are sorted in runs of RT_RUN (one block each), then
+// each entry's rank over all runs (binary searches in LDS) gives both the
+// selection (rank < m) and its output slot.  At most RT_MAX_RUNS runs.
+constexpr uint32_t RT_RUN = BLOCK * 2;
+constexpr int64_t RT_MAX_RUNS = 256;
+// Entries of the runs a rank block holds in LDS at once.
+constexpr uint32_t RT_LDS = 16384;
 
 // One task of a launch.  Pointers address the task's own first element, so a
 // launch may mix tasks from different buffers (the serving engine does).
@@ -55,7 +63,7 @@ struct TaskDesc {
   int32_t k;            // requested k (>= 0)
   uint32_t flags;       // F_* bits
   int32_t nblk;         // pass-A chunks overlapping the task
-  int32_t spare;
+  int32_t rt_runs;      // rank-tail runs the launch gives the task (0: none)
 };
 static_assert(sizeof(TaskDesc) == 64, "TaskDesc layout");
 
@@ -77,7 +85,7 @@ struct TaskCtl {
   uint32_t list_ctr;  // candidates listed by k_split
   uint32_t r_pred;    // pass A spills the composites <= (r_pred << 32 | p_pred)
   uint32_t p_pred;
-  uint32_t pad;
+  uint32_t rt;        // 1: the rank tail (k_rt_runs, k_rt_rank) finishes the task
 };
 static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 

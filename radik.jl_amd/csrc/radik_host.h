@@ -71,6 +71,7 @@ struct StageCounts {
   int64_t nch = 0;                 // pass A's chunks
   int64_t nbmap = 0;               // block-map entries (chunks + sampling blocks)
   int64_t nsplit = 0;              // k_split blocks
+  int64_t nrt = 0;                 // rank-tail runs (k_rt_runs blocks)
 };
 
 // Device + host buffers of one pipeline instance.  Every array the host
@@ -90,6 +91,7 @@ struct Pipeline {
   uint32_t* h_bmap = nullptr;   // first task of each chunk of the streaming launches
   int64_t* h_cstart = nullptr;  // pass A's chunk table: nch + 1 starts in the element space
   uint2* h_xmap = nullptr;      // k_split: task / block in task / blocks of task, per block
+  uint2* h_rmap = nullptr;      // rank tail: task / run in task / runs of task, per run
   TaskDesc* h_small = nullptr;
   TaskDesc* h_med = nullptr;
   // device
@@ -101,6 +103,7 @@ struct Pipeline {
   uint32_t* d_bmap = nullptr;
   int64_t* d_cstart = nullptr;
   uint2* d_xmap = nullptr;
+  uint2* d_rmap = nullptr;
   TaskDesc* d_small = nullptr;
   TaskDesc* d_med = nullptr;
   TaskCtl* d_ctl = nullptr;
@@ -114,6 +117,8 @@ struct Pipeline {
   size_t cap_hist = 0, cap_cand = 0;
   bool fixed = false;
   int64_t n_split = 0;  // k_split blocks (tasks longer than BIG_LEN)
+  int64_t n_rt = 0;     // rank-tail runs (tasks longer than BIG_LEN, without WITHSCALE)
+  uint32_t rt_lds = 0;  // LDS entries of a rank block: the longest list given, up to RT_LDS
   // Streaming launch shapes fixed at staging (passes A and B share one chunk
   // table, one chunk per block; sampling has equal chunks): the number of
   // chunks (= pass A's grid), the sampling grid, offsets of their maps in the

@@ -645,6 +645,14 @@ __device__ __forceinline__ void select_first(const TaskDesc& d, const uint32_t (
     }
     const uint64_t src = o.spill_all ? (uint64_t)o.cand_ctr : (uint64_t)o.nwin + o.ncand;
     o.fin = (o.m > 0 && src <= fin_cap) ? 1u : 0u;
+    // The rank tail takes a task with runs from the launch whose spill holds
+    // its m best, when its list (winners, then the spilled candidates) fits
+    // those runs and the spill region behind the spill.
+    o.rt = 0;
+    if (d.rt_runs > 0 && o.spill_all && o.m > 0) {
+      const uint64_t nl = (uint64_t)o.nwin + (o.ncand < o.cand_ctr ? o.ncand : o.cand_ctr);
+      if (nl <= (uint64_t)d.rt_runs * RT_RUN && (uint64_t)o.cand_ctr + o.nwin + o.ncand <= (uint64_t)len) o.rt = 1;
+    }
     *out = o;
   }
 }
@@ -1210,15 +1218,13 @@ __device__ __forceinline__ void write_sorted(const TaskDesc& d, Src s, uint32_t 
 // ---------------------------------------------------------------------------
 // Register-blocked bitonic sort of P2 = BLOCK * E composites, E per thread
 // (thread t holds indices t*E .. t*E+E-1), ascending.  Strides below E run in
-// registers, strides inside a wave through lane shuffles, the two widest
-// (partner in another wave) through LDS scratch xs of P2 entries.
+// registers, strides inside a wave through lane permutes (ds_bpermute), the
+// widest (partner in another wave) through LDS scratch xs of P2 entries.  A
+// cross-thread stage fetches all E partner entries before comparing any, so
+// the permutes (or LDS reads) are in flight together instead of each waiting
+// out the LDS latency on its own.  (DPP row shifts and the permlane swaps in
+// place of the permutes measured slower here.)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
-  const int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
-  const int hi = __shfl_xor((int)(uint32_t)(v >> 32), mask, 64);
-  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
-}
-
 template <int E>
 __device__ __forceinline__ void reg_bitonic(uint64_t (&a)[E], uint64_t* xs) {
   constexpr uint32_t P2 = (uint32_t)BLOCK * E;
@@ -1226,28 +1232,30 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&a)[E], uint64_t* xs) {
   for (uint32_t size = 2; size <= P2; size <<= 1) {
     for (uint32_t j = size >> 1; j >= (uint32_t)E; j >>= 1) {
       const uint32_t tj = j / E;  // partner thread = t ^ tj
+      // j and size are at least E, so every entry of a thread lies in the
+      // same half of its pair and of its size-block: one direction for all E
+      const bool keep_min = (((t * E) & j) == 0) == (((t * E) & size) == 0);
+      uint64_t y[E];
       if (tj < 64) {
+        const int src = (int)(((t & 63u) ^ tj) << 2);  // byte address of the partner lane
 #pragma unroll
         for (int l = 0; l < E; ++l) {
-          const uint64_t y = shfl_xor64(a[l], (int)tj);
-          const uint32_t i = t * E + l;
-          const bool keep_min = ((i & j) == 0) == ((i & size) == 0);
-          const uint64_t x = a[l];
-          a[l] = keep_min ? (x < y ? x : y) : (x < y ? y : x);
+          const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)a[l]);
+          const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(a[l] >> 32));
+          y[l] = ((uint64_t)hi << 32) | lo;
         }
       } else {
 #pragma unroll
         for (int l = 0; l < E; ++l) xs[t * E + l] = a[l];
         __syncthreads();
 #pragma unroll
-        for (int l = 0; l < E; ++l) {
-          const uint64_t y = xs[(t ^ tj) * E + l];
-          const uint32_t i = t * E + l;
-          const bool keep_min = ((i & j) == 0) == ((i & size) == 0);
-          const uint64_t x = a[l];
-          a[l] = keep_min ? (x < y ? x : y) : (x < y ? y : x);
-        }
+        for (int l = 0; l < E; ++l) y[l] = xs[(t ^ tj) * E + l];
         __syncthreads();
+      }
+#pragma unroll
+      for (int l = 0; l < E; ++l) {
+        const uint64_t x = a[l];
+        a[l] = ((x > y[l]) == keep_min) ? y[l] : x;
       }
     }
 #pragma unroll
@@ -1290,6 +1298,52 @@ __device__ __forceinline__ void write_slot(const TaskDesc& d, uint32_t i, uint64
   else glb(reinterpret_cast<int32_t*>(d.idx_out))[i] = (int32_t)id;
 }
 
+// Writes output slots t*E .. t*E+E-1 (thread t) from the sorted composites
+// a[] (valid below m), then the slots past BLOCK*E up to k as padding.  The
+// indices of a group of slots are gathered from idx_in before any of the
+// group's stores: a store to the outputs may alias idx_in for the compiler,
+// so a gather placed after one would wait out its own memory latency.
+template <int E>
+__device__ __forceinline__ void write_slots_regs(const TaskDesc& d, const uint64_t (&a)[E], uint32_t m) {
+  constexpr int G = E < 16 ? E : 16;
+  const uint32_t t = threadIdx.x, k = (uint32_t)d.k;
+  const uint32_t am = (d.flags & F_ASCEND) ? 0u : 0xFFFFFFFFu;
+  const bool withidx = (d.flags & F_WITHIDXIN) != 0;
+  const bool i64 = (d.flags & F_IDX64) != 0;
+#pragma unroll
+  for (int g = 0; g < E; g += G) {
+    uint32_t val[G];
+    int64_t id[G];
+#pragma unroll
+    for (int l = 0; l < G; ++l) {
+      const uint32_t i = t * E + (uint32_t)(g + l);
+      const uint64_t x = a[g + l];
+      uint32_t pos = (uint32_t)x;
+      if ((int64_t)pos >= d.len) pos = (uint32_t)(d.len - 1);  // never read outside the task
+      const bool valid = i < m;
+      val[l] = valid ? ekey_inv((uint32_t)(x >> 32) ^ am) : 0u;
+      id[l] = -1;
+      if (valid) {
+        if (withidx)
+          id[l] = i64 ? glb(reinterpret_cast<const int64_t*>(d.idx_in))[pos]
+                      : (int64_t)glb(reinterpret_cast<const int32_t*>(d.idx_in))[pos];
+        else
+          id[l] = (int64_t)pos + 1;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < G; ++l) {
+      const uint32_t i = t * E + (uint32_t)(g + l);
+      if (i < k) {
+        store_elem(d.val_out, i, val[l]);
+        if (i64) glb(reinterpret_cast<int64_t*>(d.idx_out))[i] = id[l];
+        else glb(reinterpret_cast<int32_t*>(d.idx_out))[i] = (int32_t)id[l];
+      }
+    }
+  }
+  for (uint32_t i = (uint32_t)BLOCK * E + t; i < k; i += BLOCK) write_slot(d, i, 0ull, false);
+}
+
 template <int E>
 __device__ __forceinline__ void sort_task_regs(const TaskDesc& d, uint32_t m, uint64_t* xs) {
   const bool i64 = (d.flags & F_IDX64) != 0;
@@ -1307,13 +1361,7 @@ __device__ __forceinline__ void sort_task_regs(const TaskDesc& d, uint32_t m, ui
     }
   }
   reg_bitonic<E>(a, xs);  // its widest stages synchronise the block: every read above is done
-  const uint32_t k = (uint32_t)d.k;
-#pragma unroll
-  for (int l = 0; l < E; ++l) {
-    const uint32_t i = t * E + l;
-    if (i < k) write_slot(d, i, a[l], i < m);
-  }
-  for (uint32_t i = (uint32_t)BLOCK * E + t; i < k; i += BLOCK) write_slot(d, i, 0ull, false);
+  write_slots_regs<E>(d, a, m);
 }
 
 // Output composite of a selection composite (rank key << 32 | position).
@@ -1344,13 +1392,7 @@ __device__ __forceinline__ void sort_selected_regs(const TaskDesc& d, const uint
     a[l] = (i < m) ? (same ? sel[i] : out_of_selection(d, sel[i])) : ~0ull;
   }
   reg_bitonic<E>(a, xs);
-  const uint32_t k = (uint32_t)d.k;
-#pragma unroll
-  for (int l = 0; l < E; ++l) {
-    const uint32_t i = t * E + l;
-    if (i < k) write_slot(d, i, a[l], i < m);
-  }
-  for (uint32_t i = (uint32_t)BLOCK * E + t; i < k; i += BLOCK) write_slot(d, i, 0ull, false);
+  write_slots_regs<E>(d, a, m);
 }
 
 template <int EMAX>
@@ -1394,6 +1436,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
                                                 uint64_t* __restrict__ cand) {
   extern __shared__ uint64_t sb[];
   const int t = blockIdx.x;
+  if (ctl[t].rt) return;  // sorted by the rank tail
   const TaskDesc d = tasks[t];
   const uint32_t m = ctl[t].m;
   const bool i64 = (d.flags & F_IDX64) != 0;
@@ -1430,6 +1473,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restric
                                                        uint64_t* __restrict__ cand) {
   __shared__ uint64_t xs[SORT_CHUNK];
   const int t = find_task(cbases, ntasks, blockIdx.x);
+  if (ctl[t].rt) return;  // sorted by the rank tail
   const TaskDesc d = tasks[t];
   const uint32_t m = ctl[t].m;
   const uint32_t j0 = (uint32_t)(blockIdx.x - cbases[t]) * SORT_CHUNK;
@@ -1464,6 +1508,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict_
   const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (e >= total) return;
   const int t = find_task(mbases, ntasks, e);
+  if (ctl[t].rt) return;  // sorted by the rank tail
   const TaskDesc d = tasks[t];
   const uint32_t m = ctl[t].m;
   const uint32_t j = (uint32_t)(e - mbases[t]);
@@ -1533,7 +1578,7 @@ struct FinShared {
 // behind its spill: fin tasks winners then candidates, others (winners went
 // to the output slots) only candidates.
 __device__ __forceinline__ bool split_fits(const TaskCtl& c, int64_t len) {
-  const uint64_t n = c.cand_ctr, need = c.fin ? (uint64_t)c.nwin + c.ncand : (uint64_t)c.ncand;
+  const uint64_t n = c.cand_ctr, need = (c.fin || c.rt) ? (uint64_t)c.nwin + c.ncand : (uint64_t)c.ncand;
   return c.spill_all && c.m > 0 && n <= (uint64_t)len && n + need <= (uint64_t)len;
 }
 
@@ -1885,6 +1930,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
     c = S.ctl;
     if (threadIdx.x == 0) ctl[t] = c;  // k_sort reads m
   }
+  if (c.rt) return;  // finished by the rank tail
   finish_task<EMAX>(d, c, cand, v, cap, big_len, sort_here, split != 0, S);
 }
 
@@ -1911,7 +1957,7 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
   const uint32_t per = (n + nb - 1) / nb;
   const uint32_t s = min(n, j * per), e = min(n, s + per);
   if (s >= e) return;
-  const bool fin = c.fin != 0;
+  const bool fin = c.fin != 0 || c.rt != 0;  // winners listed too (as composites)
   const uint32_t bw = c.b_win, bc = c.b_cand, nwin = c.nwin, ncand = c.ncand;
   uint64_t* __restrict__ spill = cand + d.base;
   uint64_t* __restrict__ wdst = spill + n;
@@ -1959,6 +2005,121 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
         ++cs;
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rank tail, for tasks over 2^20 elements whose spill holds their m best
+// (TaskCtl::rt, set at the selection): k_split has listed the task's winners
+// and spilled candidates as composites, list L = [nwin winners | candidates]
+// behind the spill.  The m best of the task are the m smallest of L.
+//   k_rt_runs  one block per run of RT_RUN entries of L: sorts it in place
+//              (register bitonic)
+//   k_rt_rank  one block per BLOCK entries of L: the entry's rank r over L
+//              (its place in its own run plus a binary search in every other
+//              run, the runs staged through LDS) selects it (r < m) and gives
+//              its output slot: r itself when the output order is the
+//              selection order, else the slot in the reversed order of rank
+//              keys with equal keys by position (their group bounds come from
+//              two more searches in the runs where x's rank key recurs).
+// Both take the whole chip for one task, where k_finish would sort the m
+// results in one block.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t rt_list_len(const TaskCtl& c) {
+  return c.nwin + (c.list_ctr < c.ncand ? c.list_ctr : c.ncand);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rt_runs(const TaskDesc* __restrict__ tasks,
+                                                   const TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
+                                                   const uint2* __restrict__ rmap) {
+  constexpr int E = (int)(RT_RUN / BLOCK);
+  __shared__ uint64_t xs[RT_RUN];
+  const uint2 tj = rmap[blockIdx.x];
+  const uint32_t t = tj.x, j = tj.y & 0xFFFFu;
+  const TaskCtl c = ctl[t];
+  if (!c.rt) return;
+  const uint32_t nl = rt_list_len(c);
+  const uint32_t s0 = j * RT_RUN;
+  if (s0 >= nl) return;
+  uint64_t* __restrict__ L = cand + tasks[t].base + c.cand_ctr;
+  uint64_t a[E];
+#pragma unroll
+  for (int l = 0; l < E; ++l) {
+    const uint32_t i = s0 + threadIdx.x * E + (uint32_t)l;
+    a[l] = i < nl ? L[i] : ~0ull;
+  }
+  reg_bitonic<E>(a, xs);
+#pragma unroll
+  for (int l = 0; l < E; ++l) {
+    const uint32_t i = s0 + threadIdx.x * E + (uint32_t)l;
+    if (i < nl) L[i] = a[l];
+  }
+}
+
+// Entries of the sorted run `run` (rl of them) below v.
+__device__ __forceinline__ uint32_t run_below(const uint64_t* run, uint32_t rl, uint64_t v) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t step = RT_RUN; step > 0; step >>= 1)
+    if (pos + step <= rl && run[pos + step - 1] < v) pos += step;
+  return pos;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ tasks,
+                                                   const TaskCtl* __restrict__ ctl, const uint64_t* __restrict__ cand,
+                                                   const uint2* __restrict__ rmap, uint32_t lds_entries) {
+  extern __shared__ uint64_t lr[];  // lds_entries, a multiple of RT_RUN
+  constexpr uint32_t HALVES = RT_RUN / BLOCK;
+  const uint2 tj = rmap[blockIdx.x / HALVES];
+  const uint32_t t = tj.x, j = tj.y & 0xFFFFu;
+  const TaskCtl c = ctl[t];
+  if (!c.rt) return;
+  const TaskDesc d = tasks[t];
+  const uint32_t nl = rt_list_len(c), m = c.m;
+  const uint32_t q0 = j * RT_RUN + (blockIdx.x % HALVES) * BLOCK;
+  if (q0 == 0)  // the task's first block pads the slots past m
+    for (uint32_t i = m + threadIdx.x; i < (uint32_t)d.k; i += BLOCK) write_slot(d, i, 0ull, false);
+  if (q0 >= nl) return;
+  const uint64_t* __restrict__ L = cand + d.base + c.cand_ctr;
+  const uint32_t qi = q0 + threadIdx.x;
+  const bool valid = qi < nl;
+  const uint64_t x = valid ? L[qi] : ~0ull;
+  const uint32_t own = qi / RT_RUN, runs = (nl + RT_RUN - 1) / RT_RUN;
+  const bool same = same_order(d);
+  const uint32_t rk = (uint32_t)(x >> 32);
+  const uint64_t key_lo = (uint64_t)rk << 32, key_hi = key_lo | 0xFFFFFFFFull;  // rank key rk: [key_lo, key_hi)
+  uint32_t rank = 0, glo = 0, ghi = 0;  // entries below x; with a rank key below rk; at most rk
+  const uint32_t per = lds_entries / RT_RUN;
+  for (uint32_t r0 = 0; r0 < runs; r0 += per) {
+    const uint32_t r1 = min(runs, r0 + per);
+    const uint32_t e0 = r0 * RT_RUN, e1 = min(nl, r1 * RT_RUN);
+    __syncthreads();  // the previous group's searches are done
+    for (uint32_t i = e0 + threadIdx.x; i < e1; i += BLOCK) lr[i - e0] = L[i];
+    __syncthreads();
+    for (uint32_t r = r0; r < r1; ++r) {
+      const uint64_t* run = lr + (r - r0) * RT_RUN;
+      const uint32_t rl = min(RT_RUN, nl - r * RT_RUN);
+      const bool mine = valid && r == own;
+      const uint32_t lb = mine ? qi - r * RT_RUN : run_below(run, rl, x);
+      rank += lb;
+      if (!same) {
+        // entries of this run with rank key rk lie next to x's place
+        const uint32_t nx = mine ? lb + 1 : lb;
+        const bool tie = (lb > 0 && (uint32_t)(run[lb - 1] >> 32) == rk) ||
+                         (nx < rl && (uint32_t)(run[nx] >> 32) == rk);
+        if (tie) {
+          glo += run_below(run, rl, key_lo);
+          ghi += run_below(run, rl, key_hi);  // (x < key_hi: positions stay below 2^32 - 1)
+        } else {
+          glo += lb;
+          ghi += nx;
+        }
+      }
+    }
+  }
+  if (valid && rank < m) {
+    const uint32_t slot = same ? rank : m - min(m, ghi) + (rank - glo);
+    write_slot(d, slot, same ? x : out_of_selection(d, x), true);
   }
 }
 

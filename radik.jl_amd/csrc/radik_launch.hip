@@ -149,6 +149,17 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
+    if (p.n_rt > 0) {
+      // rank tail of the tasks over BIG_LEN whose spill holds their m best
+      hipLaunchKernelGGL(k_rt_runs, dim3((unsigned)p.n_rt), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
+                         p.d_rmap);
+      RADIK_HIP(hipGetLastError());
+      const uint32_t fit = (uint32_t)(f.lds_per_block / 8 / RT_RUN) * RT_RUN;
+      const uint32_t le = std::max<uint32_t>(RT_RUN, std::min<uint32_t>(p.rt_lds, fit));
+      hipLaunchKernelGGL(k_rt_rank, dim3((unsigned)(p.n_rt * (RT_RUN / BLOCK))), dim3(BLOCK), (size_t)le * 8, s,
+                         p.d_large, p.d_ctl, (const uint64_t*)p.d_cand, p.d_rmap, le);
+      RADIK_HIP(hipGetLastError());
+    }
     {
       const size_t lds = ((size_t)cap + sel) * 8;
       const dim3 g((unsigned)p.n_large), b(BLOCK);
@@ -229,6 +240,7 @@ int RADIK_PASTE(launch_setup_, RADIK_KNS)(size_t lds_budget, int* hist_occ) {
               hipFuncSetAttribute((const void*)k_finish<16>, A, fin_want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_finish<32>, A, fin_want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_finish<64>, A, fin_want) == hipSuccess &&
+              hipFuncSetAttribute((const void*)k_rt_rank, A, want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_sort<32>, A, want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_sort<64>, A, want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_small, A, want - 20 * 1024) == hipSuccess;

@@ -123,7 +123,7 @@ static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // Offsets of the arena's arrays for counts c: the uploaded part (everything
 // the host builds), then the device-only control blocks.
 struct ArenaLayout {
-  size_t large, bases, sbases, cbases, mbases, cstart, bmap, xmap, small, med, upload, ctl, total;
+  size_t large, bases, sbases, cbases, mbases, cstart, bmap, xmap, rmap, small, med, upload, ctl, total;
 };
 
 static ArenaLayout arena_layout(const StageCounts& c) {
@@ -138,6 +138,7 @@ static ArenaLayout arena_layout(const StageCounts& c) {
   L.cstart = o; o = align256(o + (size_t)(c.nch + 1) * sizeof(int64_t));
   L.bmap = o; o = align256(o + (size_t)c.nbmap * sizeof(uint32_t));
   L.xmap = o; o = align256(o + (size_t)c.nsplit * sizeof(uint2));
+  L.rmap = o; o = align256(o + (size_t)c.nrt * sizeof(uint2));
   L.small = o; o = align256(o + (size_t)c.ns * sizeof(TaskDesc));
   L.med = o; o = align256(o + (size_t)c.nm * sizeof(TaskDesc));
   L.upload = o;
@@ -201,6 +202,7 @@ int Pipeline::reserve(const StageCounts& c, int64_t tot, hipStream_t s) {
   h_cstart = (int64_t*)(h_arena + L.cstart);
   h_bmap = (uint32_t*)(h_arena + L.bmap);
   h_xmap = (uint2*)(h_arena + L.xmap);
+  h_rmap = (uint2*)(h_arena + L.rmap);
   h_small = (TaskDesc*)(h_arena + L.small);
   h_med = (TaskDesc*)(h_arena + L.med);
   d_large = (TaskDesc*)(d_arena + L.large);
@@ -211,6 +213,7 @@ int Pipeline::reserve(const StageCounts& c, int64_t tot, hipStream_t s) {
   d_cstart = (int64_t*)(d_arena + L.cstart);
   d_bmap = (uint32_t*)(d_arena + L.bmap);
   d_xmap = (uint2*)(d_arena + L.xmap);
+  d_rmap = (uint2*)(d_arena + L.rmap);
   d_small = (TaskDesc*)(d_arena + L.small);
   d_med = (TaskDesc*)(d_arena + L.med);
   d_ctl = (TaskCtl*)(d_arena + L.ctl);
@@ -347,6 +350,16 @@ static ChunkShape chunk_shape(const DevFacts& f, const LaunchOpts& o) {
 // longer than BIG_LEN.
 static inline int64_t split_blocks(int64_t len) { return len > BIG_LEN ? std::min<int64_t>(256, len >> 20) : 0; }
 
+// Rank-tail runs of a task (tasks longer than BIG_LEN without WITHSCALE):
+// room for a list of 4 m entries (at least 32 runs), up to RT_MAX_RUNS.  A
+// task whose winners and candidates exceed it finishes in k_finish.
+static inline int64_t rt_runs(int64_t len, int64_t k, uint32_t flags) {
+  const int64_t m = k < len ? k : len;
+  if (len <= BIG_LEN || m <= 0 || (flags & F_WITHSCALE)) return 0;
+  const int64_t want = std::min<int64_t>(len, std::max<int64_t>(4 * m, 32 * (int64_t)RT_RUN));
+  return std::min<int64_t>(RT_MAX_RUNS, (want + RT_RUN - 1) / RT_RUN);
+}
+
 // Long tasks are laid out predicted first, then the ones pass B always
 // re-reads, so pass B can cover just the latter's elements densely.
 int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f, const LaunchOpts& o,
@@ -367,6 +380,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
       tot += len;
       groups += g;
       cnt.nsplit += split_blocks(len);
+      cnt.nrt += rt_runs(len, k, tasks[t].flags);
     }
   }
   const ChunkShape cs = chunk_shape(f, o);
@@ -397,7 +411,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     d.k = (int32_t)ts.k;
     d.flags = ts.flags;
     d.nblk = 0;
-    d.spare = 0;
+    d.rt_runs = 0;
     if (shrt) {
       d.base = 0;
       tot_small += ts.len;
@@ -443,7 +457,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
     int64_t c = 0;
     for (int64_t t = 0; t < nl; ++t) {
       const int64_t b0 = p.h_bases[t], len = p.h_large[t].len;
-      p.h_large[t].spare = 0;
+      p.h_large[t].rt_runs = 0;
       if (len <= 0) {
         p.h_large[t].nblk = 0;
         continue;
@@ -458,12 +472,19 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   p.off_s = p.off_h + nch;
   p.n_bmap = cnt.nbmap;
   fill_chunk_map(p.h_bmap + p.off_h, nch, p.h_cstart, p.h_bases, nl);
-  int64_t ix = 0;
+  int64_t ix = 0, ir = 0, rt_max = 0;
   for (int64_t t = 0; t < nl; ++t) {
-    const uint32_t nb = (uint32_t)split_blocks(p.h_large[t].len);
+    TaskDesc& d = p.h_large[t];
+    const uint32_t nb = (uint32_t)split_blocks(d.len);
     for (uint32_t j = 0; j < nb; ++j) p.h_xmap[ix++] = make_uint2((uint32_t)t, j | (nb << 16));
+    const uint32_t nr = (uint32_t)rt_runs(d.len, d.k, d.flags);
+    d.rt_runs = (int32_t)nr;
+    for (uint32_t j = 0; j < nr; ++j) p.h_rmap[ir++] = make_uint2((uint32_t)t, j | (nr << 16));
+    if ((int64_t)nr > rt_max) rt_max = nr;
   }
   p.n_split = cnt.nsplit;
+  p.n_rt = cnt.nrt;
+  p.rt_lds = (uint32_t)std::min<int64_t>(RT_LDS, rt_max * RT_RUN);
   fill_block_map(p.h_bmap + p.off_s, p.grid_s, 0, SAMPLE_GROUPS_PER_BLOCK, p.h_sbases, nl);
   p.total_runs = runs;
   p.total_m = msum;
