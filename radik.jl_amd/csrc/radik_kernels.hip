@@ -440,6 +440,9 @@ __global__ __launch_bounds__(BLOCK) void k_sample(const TaskDesc* __restrict__ t
 // positions too, so an exact tie across the threshold spills only about as
 // many entries as needed.  Clears the sample bins.
 // ---------------------------------------------------------------------------
+template <int E>
+__device__ __forceinline__ void reg_bitonic(uint64_t (&a)[E], uint64_t* xs);
+
 constexpr uint32_t PRED_MIN = 32;
 constexpr uint32_t PRED_CAP = 1024;
 static_assert(PRED_CAP == BLOCK * 4, "the prediction ranks PRED_CAP composites, 4 per thread");
@@ -529,25 +532,21 @@ __device__ __forceinline__ void predict_task(int t, const TaskDesc* __restrict__
       __syncthreads();
       if (nsel == cnt && need > before && need - before <= cnt) {
         // the q-th smallest sampled composite is the qi-th of digit b: sort
-        // the digit's sampled composites in registers (PRED_CAP = BLOCK * 4)
+        // the digit's sampled composites in registers (PRED_CAP = BLOCK * 4,
+        // +inf padding behind them) and take entry qi - 1
         const uint32_t qi = need - before;
-        // the entry with exactly qi - 1 smaller ones (composites are
-        // distinct): each thread ranks its (up to) 4 entries against all
         uint64_t x4[4];
-        uint32_t r4[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
-          const uint32_t i = threadIdx.x + (uint32_t)l * BLOCK;
+          const uint32_t i = threadIdx.x * 4u + (uint32_t)l;
           x4[l] = i < cnt ? sv[i] : ~0ull;
         }
-        for (uint32_t j = 0; j < cnt; ++j) {
-          const uint64_t y = sv[j];
-#pragma unroll
-          for (int l = 0; l < 4; ++l) r4[l] += y < x4[l] ? 1u : 0u;
-        }
+        __syncthreads();  // sv is the sort's scratch from here
+        reg_bitonic<4>(x4, sv);
+        const uint32_t qq = qi - 1u;
 #pragma unroll
         for (int l = 0; l < 4; ++l)
-          if (threadIdx.x + (uint32_t)l * BLOCK < cnt && r4[l] == qi - 1) P.qv = x4[l];
+          if (threadIdx.x == qq / 4u && (uint32_t)l == qq % 4u) P.qv = x4[l];
         __syncthreads();
         const uint64_t Ub = P.qv;
         rp = (uint32_t)(Ub >> 32);
@@ -965,13 +964,31 @@ __device__ __forceinline__ void pass_b_segment(const TaskDesc& d, int64_t s, int
   stage_flush(scd, cctr, write_cand);
 }
 
+// Rank-tail tasks whose list (winners and spilled candidates) is cut at the
+// second-digit bin of the m-th best: those with candidates and a list longer
+// than two runs.
+__device__ __forceinline__ bool rt_cut(const TaskCtl& c) {
+  return c.rt && c.b_cand != NO_BIN && (uint64_t)c.nwin + min(c.ncand, c.cand_ctr) > 2ull * RT_RUN;
+}
+
+// Pass B proper: one chunk of pass A's table per block (above).  The same
+// blocks count, for the rank-tail tasks with a cut (rt_cut), the second
+// digit (rank-key bits [19:8]) of their spilled candidates (first digit
+// b_cand) into the task's sample-histogram slot (free after the prediction;
+// k_rt_runs clears it): each block takes the part of the spill proportional
+// to its part of the task.  k_split cuts the candidates it lists at the
+// second-digit bin holding the m-th best, so the rank tail sorts about m
+// entries instead of the whole candidate digit.  (The rank tail's list
+// before the cut fits its runs, so a task adds at most RT_MAX_RUNS * RT_RUN
+// counts, merged per wave.)
 template <int UNR>
 __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ bases, int ntasks,
                                                    const int64_t* __restrict__ cstart,
                                                    TaskCtl* __restrict__ ctl,
                                                    uint64_t* __restrict__ cand,
-                                                   const uint32_t* __restrict__ cmap, int64_t min_len) {
+                                                   const uint32_t* __restrict__ cmap, int64_t min_len,
+                                                   uint32_t* __restrict__ shist) {
   __shared__ uint64_t stage_w[BLOCK / 64][STAGE];
   __shared__ uint64_t stage_c[BLOCK / 64][STAGE];
   __shared__ uint32_t stage_n[2][BLOCK / 64];
@@ -997,6 +1014,16 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
       uint64_t* cb = cand + d.base;
       if (task_scales(d)) pass_b_segment<UNR, true>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
       else pass_b_segment<UNR, false>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
+    } else if (rt_cut(c)) {
+      const uint64_t n = c.cand_ctr;  // (<= len: the rank tail's list fits behind it)
+      const uint32_t ps = (uint32_t)((uint64_t)(g0 - tb) * n / (uint64_t)d.len);
+      const uint32_t pe = (uint32_t)((uint64_t)(se - tb) * n / (uint64_t)d.len);
+      const uint64_t* __restrict__ spill = cand + d.base;
+      uint32_t* __restrict__ h2 = shist + (size_t)t * HIST_BINS;
+      for (uint32_t i = ps + threadIdx.x; i < pe; i += BLOCK) {
+        const uint64_t x = spill[i];
+        if ((uint32_t)(x >> 52) == c.b_cand) hist_add(h2, (uint32_t)(x >> 40) & 0xFFFu);
+      }
     }
     if (se >= g1) break;
     g0 = se;
@@ -1935,30 +1962,56 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
 }
 
 // ---------------------------------------------------------------------------
+// Part j of nb of a task's spill of n entries for a block of k_split: at
+// least one full tile of BLOCK * LOADS_IN_FLIGHT entries per block, so a
+// short spill takes few blocks.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void spill_part(uint32_t n, uint32_t j, uint32_t nb, uint32_t& s, uint32_t& e) {
+  const uint32_t per = max((n + nb - 1) / nb, (uint32_t)BLOCK * LOADS_IN_FLIGHT);
+  s = (uint32_t)min((uint64_t)n, (uint64_t)j * per);
+  e = (uint32_t)min((uint64_t)n, (uint64_t)s + per);
+}
+
+// ---------------------------------------------------------------------------
 // Split of the spills of long tasks (longer than big_len) over several blocks
 // ahead of k_finish, so one block does not read a long spill alone: winners
 // and candidates listed behind the spill (split_fits; fin tasks winners then
 // candidates, others candidates only with the winners in the output slots),
 // placed per tile of BLOCK * LOADS_IN_FLIGHT entries by one block scan and
-// one global atomic per list.  xmap: task, then block index in the task
-// (low 16 bits) and blocks of the task (high 16 bits), of each block.
+// one global atomic per list.  Rank-tail tasks list only the candidates up
+// to the second-digit bin (counted by k_compact) holding the m-th best.
+// xmap: task, then block index in the task (low 16 bits) and blocks of the
+// task (high 16 bits), of each block.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ tasks, TaskCtl* __restrict__ ctl,
-                                                 uint64_t* __restrict__ cand, const uint2* __restrict__ xmap) {
+                                                 uint64_t* __restrict__ cand, const uint2* __restrict__ xmap,
+                                                 const uint32_t* __restrict__ shist) {
   constexpr int U = LOADS_IN_FLIGHT;
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t slot0[2];
+  __shared__ uint32_t res[3];
   const uint2 tj = xmap[blockIdx.x];
   const uint32_t t = tj.x, j = tj.y & 0xFFFFu, nb = tj.y >> 16;
   const TaskCtl c = ctl[t];
   const TaskDesc d = tasks[t];
   if (!split_fits(c, d.len)) return;
   const uint32_t n = c.cand_ctr;
-  const uint32_t per = (n + nb - 1) / nb;
-  const uint32_t s = min(n, j * per), e = min(n, s + per);
+  uint32_t s, e;
+  spill_part(n, j, nb, s, e);
   if (s >= e) return;
   const bool fin = c.fin != 0 || c.rt != 0;  // winners listed too (as composites)
   const uint32_t bw = c.b_win, bc = c.b_cand, nwin = c.nwin, ncand = c.ncand;
+  uint32_t rl = 0xFFFFFFFFu;  // candidates listed: rank keys up to rl
+  if (rt_cut(c)) {
+    // the second-digit bin of the krem-th spilled candidate (the spill is a
+    // prefix of the task's composite order holding its m best, so its
+    // candidates up to that bin are too, with at least krem of them)
+    uint32_t c2[16];
+    load_hist(shist + (size_t)t * HIST_BINS, c2);
+    if (threadIdx.x == 0) res[0] = NO_BIN;
+    block_find_bin<16>(c2, c.krem, wsum, res);
+    if (res[0] < (uint32_t)HIST_BINS) rl = (bc << 20) | (res[0] << 8) | 0xFFu;
+  }
   uint64_t* __restrict__ spill = cand + d.base;
   uint64_t* __restrict__ wdst = spill + n;
   uint64_t* __restrict__ cdst = spill + n + (fin ? nwin : 0u);
@@ -1975,7 +2028,7 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
       if (tile + (uint32_t)u * BLOCK + threadIdx.x >= e) continue;
       const uint32_t dg = (uint32_t)(xs[u] >> 52);
       if (dg < bw) wmask |= 1u << u;
-      else if (dg == bc) cmask |= 1u << u;
+      else if (dg == bc && (uint32_t)(xs[u] >> 32) <= rl) cmask |= 1u << u;
     }
     const uint32_t mine = (uint32_t)__builtin_popcount(wmask) | ((uint32_t)__builtin_popcount(cmask) << 16);
     const uint32_t incl = block_incl_scan(mine, wsum);
@@ -2025,19 +2078,25 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
 // Both take the whole chip for one task, where k_finish would sort the m
 // results in one block.
 // ---------------------------------------------------------------------------
+// Loads in flight per thread while a rank block fills its LDS with runs, and
+// runs searched together.
+constexpr int RT_FILL = 16;
+constexpr int RT_ILP = 8;
+
 __device__ __forceinline__ uint32_t rt_list_len(const TaskCtl& c) {
   return c.nwin + (c.list_ctr < c.ncand ? c.list_ctr : c.ncand);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_rt_runs(const TaskDesc* __restrict__ tasks,
                                                    const TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
-                                                   const uint2* __restrict__ rmap) {
+                                                   const uint2* __restrict__ rmap, uint32_t* __restrict__ shist) {
   constexpr int E = (int)(RT_RUN / BLOCK);
   __shared__ uint64_t xs[RT_RUN];
   const uint2 tj = rmap[blockIdx.x];
   const uint32_t t = tj.x, j = tj.y & 0xFFFFu;
   const TaskCtl c = ctl[t];
   if (!c.rt) return;
+  if (j == 0 && rt_cut(c)) clear_hist(shist + (size_t)t * HIST_BINS);  // k_compact's, for the next launch
   const uint32_t nl = rt_list_len(c);
   const uint32_t s0 = j * RT_RUN;
   if (s0 >= nl) return;
@@ -2094,25 +2153,65 @@ __global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ 
     const uint32_t r1 = min(runs, r0 + per);
     const uint32_t e0 = r0 * RT_RUN, e1 = min(nl, r1 * RT_RUN);
     __syncthreads();  // the previous group's searches are done
-    for (uint32_t i = e0 + threadIdx.x; i < e1; i += BLOCK) lr[i - e0] = L[i];
+    // RT_FILL loads in flight per thread before their LDS stores
+    for (uint32_t i0 = e0; i0 < e1; i0 += (uint32_t)BLOCK * RT_FILL) {
+      uint64_t tmp[RT_FILL];
+#pragma unroll
+      for (int u = 0; u < RT_FILL; ++u) {
+        const uint32_t i = i0 + (uint32_t)u * BLOCK + threadIdx.x;
+        tmp[u] = i < e1 ? L[i] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < RT_FILL; ++u) {
+        const uint32_t i = i0 + (uint32_t)u * BLOCK + threadIdx.x;
+        if (i < e1) lr[i - e0] = tmp[u];
+      }
+    }
     __syncthreads();
-    for (uint32_t r = r0; r < r1; ++r) {
-      const uint64_t* run = lr + (r - r0) * RT_RUN;
-      const uint32_t rl = min(RT_RUN, nl - r * RT_RUN);
-      const bool mine = valid && r == own;
-      const uint32_t lb = mine ? qi - r * RT_RUN : run_below(run, rl, x);
-      rank += lb;
-      if (!same) {
-        // entries of this run with rank key rk lie next to x's place
-        const uint32_t nx = mine ? lb + 1 : lb;
-        const bool tie = (lb > 0 && (uint32_t)(run[lb - 1] >> 32) == rk) ||
-                         (nx < rl && (uint32_t)(run[nx] >> 32) == rk);
-        if (tie) {
-          glo += run_below(run, rl, key_lo);
-          ghi += run_below(run, rl, key_hi);  // (x < key_hi: positions stay below 2^32 - 1)
-        } else {
-          glo += lb;
-          ghi += nx;
+    for (uint32_t r = r0; r < r1; r += RT_ILP) {
+      // the binary searches of RT_ILP runs interleaved, step by step, so
+      // their LDS reads are in flight together
+      // (branch-free: a run past r1 has length 0 and probes entry 0)
+      uint32_t lbs[RT_ILP], rls[RT_ILP], bas[RT_ILP];
+#pragma unroll
+      for (int q = 0; q < RT_ILP; ++q) {
+        const uint32_t rr = r + (uint32_t)q;
+        const bool ok = rr < r1;
+        rls[q] = ok ? min(RT_RUN, nl - rr * RT_RUN) : 0u;
+        bas[q] = ok ? (rr - r0) * RT_RUN : 0u;
+        lbs[q] = 0;
+      }
+#pragma unroll
+      for (uint32_t step = RT_RUN; step > 0; step >>= 1) {
+#pragma unroll
+        for (int q = 0; q < RT_ILP; ++q) {
+          const uint32_t pr = lbs[q] + step;
+          const bool in = pr <= rls[q];
+          const uint64_t y = lr[bas[q] + (in ? pr - 1u : 0u)];
+          lbs[q] = (in && y < x) ? pr : lbs[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RT_ILP; ++q) {
+        const uint32_t rr = r + (uint32_t)q;
+        if (rr >= r1) break;
+        const uint64_t* run = lr + (rr - r0) * RT_RUN;
+        const uint32_t rl = min(RT_RUN, nl - rr * RT_RUN);
+        const bool mine = valid && rr == own;
+        const uint32_t lb = mine ? qi - rr * RT_RUN : lbs[q];
+        rank += lb;
+        if (!same) {
+          // entries of this run with rank key rk lie next to x's place
+          const uint32_t nx = mine ? lb + 1 : lb;
+          const bool tie = (lb > 0 && (uint32_t)(run[lb - 1] >> 32) == rk) ||
+                           (nx < rl && (uint32_t)(run[nx] >> 32) == rk);
+          if (tie) {
+            glo += run_below(run, rl, key_lo);
+            ghi += run_below(run, rl, key_hi);  // (x < key_hi: positions stay below 2^32 - 1)
+          } else {
+            glo += lb;
+            ghi += nx;
+          }
         }
       }
     }

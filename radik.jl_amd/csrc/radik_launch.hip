@@ -38,7 +38,7 @@ template <int UNR>
 static void launch_stream_kernels_compact(const Pipeline& p, hipStream_t s) {
   if (p.n_chunks <= 0) return;
   hipLaunchKernelGGL(k_compact<UNR>, dim3((unsigned)p.n_chunks), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                     (int)p.n_large, p.d_cstart, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN);
+                     (int)p.n_large, p.d_cstart, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN, p.d_shist);
 }
 
 // k_small and k_medium (independent of the long tasks).
@@ -145,14 +145,14 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
       }
       RADIK_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_split, dim3((unsigned)p.n_split), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
-                         p.d_xmap);
+                         p.d_xmap, p.d_shist);
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
     if (p.n_rt > 0) {
       // rank tail of the tasks over BIG_LEN whose spill holds their m best
       hipLaunchKernelGGL(k_rt_runs, dim3((unsigned)p.n_rt), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
-                         p.d_rmap);
+                         p.d_rmap, p.d_shist);
       RADIK_HIP(hipGetLastError());
       const uint32_t fit = (uint32_t)(f.lds_per_block / 8 / RT_RUN) * RT_RUN;
       const uint32_t le = std::max<uint32_t>(RT_RUN, std::min<uint32_t>(p.rt_lds, fit));

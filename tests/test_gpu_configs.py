@@ -190,3 +190,49 @@ def test_reread_over_several_blocks_gathered_in_lds():
             radik.topk_radix_select_(vo, io, ws, d, di, [n], k, LARGEST=bool(flags & O.LARGEST),
                                      ASCEND=bool(flags & O.ASCEND), WITHIDXIN=True)
             same(vo.to_host(), io.to_host(), ev.T, ei.T, f"prediction miss k{k} f{flags}")
+
+
+# ----------------------------------------------------------------- rank tail
+@pytest.mark.parametrize("k", [100, 1000, 20000])
+def test_rank_tail_second_digit_cut(k):
+    # Tasks over 2^20 elements whose spill holds their m best finish in the
+    # rank tail.  When the m-th best lies in the predicted first digit, the
+    # tail lists only the candidates up to the second-digit bin holding it.
+    # Cases: smooth data; an exact tie straddling the m-th best (the cut
+    # takes its whole bin); a tie of 5 % of the task in the threshold digit
+    # (the list exceeds the tail, which falls back); two big tasks at once.
+    n = 1 << 22
+    rng = np.random.default_rng(4242 + k)
+    base = rng.standard_normal(n).astype(np.float32)
+    cases = {"smooth": (base, [n])}
+    x = base.copy()
+    v = np.partition(base, n - k)[n - k]  # the k-th largest
+    x[rng.integers(0, n, 3000)] = v
+    cases["tie_at_threshold"] = (x, [n])
+    x = base.copy()
+    x[rng.random(n) < 0.05] = np.float32(2.75)
+    cases["tie_heavy"] = (x, [n])
+    cases["two_tasks"] = (base, [(1 << 21) + 777, (1 << 21) - 999])
+    for name, (x, lens) in cases.items():
+        T = len(lens)
+        lens = [int(a) for a in lens]
+        xx = x[:sum(lens)]
+        d = DeviceArray.from_host(xx)
+        perm = rng.permutation(sum(lens)).astype(np.int32)
+        di = DeviceArray.from_host(perm)
+        for flags in (O.LARGEST | O.ASCEND, O.LARGEST, O.ASCEND):
+            largest, rev = bool(flags & O.LARGEST), not (flags & O.ASCEND)
+            ev, ei = O.topk_exact(xx, lens, k, flags, idx_in=perm, threads=O.max_threads())
+            say("rank tail", name, k, flags)
+            vo = DeviceArray.zeros((k, T), np.float32)
+            io = DeviceArray.zeros((k, T), np.int32)
+            ws = radik.RadiKWorkspace(sum(lens), T)
+            radik.topk_radix_select_(vo, io, ws, d, di, lens, k, LARGEST=largest, ASCEND=not rev, WITHIDXIN=True)
+            same(vo.to_host(), io.to_host(), ev.T, ei.T, f"rank tail {name} k{k} f{flags} ws")
+            del ws
+            p = radik.Plan(lens, k, largest=largest, rev=rev, indices_dtype=np.int32)
+            for rep in range(2):
+                vo.fill(np.float32(-7))
+                p.execute(vo, io, d, di)
+                radik.synchronize()
+                same(vo.to_host(), io.to_host(), ev.T, ei.T, f"rank tail {name} k{k} f{flags} plan rep{rep}")
