@@ -151,7 +151,9 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * (16/32/64), "blocks_per_cu", "small_cap", "refine_cap", "predict" (0/1)
  * as in radik_plan_set_options, and "medium" (0/1: tasks of 2^11..2^16
  * elements run whole in one block beside the streaming passes, or join
- * the streaming passes and are re-read by the finishing kernel). */
+ * the streaming passes and are re-read by the finishing kernel), and
+ * "fork_late" (0/1: the short and medium tasks' kernels start with the
+ * launch, or once the long tasks' prediction is done). */
 int radik_plan_set_option(radik_plan *plan, const char *name, int64_t value);
 /* Per-kernel timing: when enabled, execute runs eagerly (every kernel on the
  * plan's stream) and records HIP events between the phases; radik_plan_timing

@@ -443,6 +443,8 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
   if (n == "medium") {
     p->opts.medium = value != 0;
 
+  } else if (n == "fork_late") {
+    p->opts.fork_late = value != 0;
   } else if (n == "predict") {
     p->opts.predict = value != 0;
   } else if (n == "blocks_per_cu") {
@@ -463,6 +465,7 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
     p->opts.refine_cap = keep.refine_cap;
     p->opts.predict = keep.predict;
     p->opts.medium = keep.medium;
+    p->opts.fork_late = keep.fork_late;
   } else {
     return set_error(RADIK_EINVAL, "unknown plan option '%s'", name);
   }

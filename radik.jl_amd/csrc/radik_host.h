@@ -56,6 +56,7 @@ struct LaunchOpts {
   uint32_t refine_cap = 0;  // candidates held in LDS by k_finish (0: device default)
   bool predict = true;      // sample long tasks and spill in pass A
   bool medium = true;       // medium tasks to k_medium (else the streaming passes)
+  bool fork_late = false;   // short/medium tasks start after the prediction (else with the launch)
   int key_type = KEY_F32;   // element type of every task of the launch (KEY_*)
 };
 inline uint32_t eff_small_cap(const DevFacts& f, const LaunchOpts& o) {

@@ -712,110 +712,125 @@ __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ t
 // same read also spills every composite (rank << 32 | position) at or below
 // the predicted bound (r_pred, p_pred) into the task's spill region.
 // ---------------------------------------------------------------------------
+// Pass A over elements [s, e) of task d.  Every loop below runs a trip
+// count that is the same for all lanes of a wave (per-lane validity instead
+// of per-lane bounds), so the fill n of the wave's stage (STAGE entries of
+// LDS) is wave-uniform and lives in a scalar register; the vector body issues
+// the next iteration's UNR loads before it processes the current ones, and
+// tests the spill bound once per lane and iteration (the minimum rank key of
+// the lane's elements) before placing any hit.
 template <int UNR, bool SCALE, bool SPILL, bool POS>
 __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int64_t e, float sc, uint32_t xm,
                                                uint32_t rp, uint32_t pp, uint32_t* lh, uint32_t* ctr,
-                                               uint64_t* __restrict__ spill, const WaveStage& st) {
+                                               uint64_t* __restrict__ spill, uint64_t* stage) {
+  static_assert((int)BODY_ALIGN <= BLOCK, "the scalar head takes one step");
+  const uint32_t slen = (uint32_t)d.len;
+  const uint32_t lane = lane_id();
+  uint32_t n = 0;  // entries in this wave's stage
+  // the stage to the task's spill region (one atomic per flush)
+  auto flush = [&]() {
+    lds_wave_sync();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(ctr, n);
+    base = __builtin_amdgcn_readfirstlane(base);
+    for (uint32_t i = lane; i < n; i += 64) {
+      const uint32_t slot = base + i;
+      if (slot < slen) glb(spill)[slot] = stage[i];
+    }
+    lds_wave_sync();
+    n = 0;
+  };
+  // the composites of the lanes with h enter the stage (whole wave)
+  auto append = [&](bool h, uint32_t rk, uint32_t pos) {
+    const uint64_t mk = ballot(h);
+    if (mk) {
+      if (n > STAGE - 64) flush();
+      if (h) stage[n + lanes_below(mk)] = ((uint64_t)rk << 32) | pos;
+      n += (uint32_t)__builtin_popcountll(mk);
+    }
+  };
   // spilled: composites (rank << 32 | position) at or below (rp << 32 | pp);
-  // without POS the bound takes every position of rank rp (pp = 0xFFFFFFFF)
+  // without POS the bound takes every position of rank rp
   auto spills = [&](uint32_t rk, uint32_t pos) {
     if (POS) return rk < rp || (rk == rp && pos <= pp);
     (void)pos;
     return rk <= rp;
   };
-  const uint32_t slen = (uint32_t)d.len;
-  auto write = [&](uint32_t slot, uint64_t v) {
-    if (slot < slen) glb(spill)[slot] = v;
+  // one element per lane (ok: the lane holds one)
+  auto one = [&](bool ok, int64_t i) {
+    uint32_t rk = 0;
+    if (ok) {
+      rk = rank_bits<SCALE>(load_elem(d.in, i), sc, xm);
+      atomicAdd(&lh[rk >> 20], 1u);
+    }
+    if (SPILL) append(ok && spills(rk, (uint32_t)i), rk, (uint32_t)i);
   };
-  auto one = [&](uint32_t x, uint32_t pos) {
-    const uint32_t rk = rank_bits<SCALE>(x, sc, xm);
-    atomicAdd(&lh[rk >> 20], 1u);
-    if (SPILL) stage_append(st, spills(rk, pos), ((uint64_t)rk << 32) | pos, ctr, write);
-  };
-  // Scalar head up to 128-byte alignment and tail, as in stream_range; the
-  // vector body spills per iteration: the UNR*EPV hits of each lane (a bit
-  // mask) enter the wave's stage together, placed by one ballot per element
-  // slot, so a rare spill costs a few scalar instructions per iteration.
-  constexpr int NB = EPV * UNR;  // elements per lane per iteration (<= 32)
-  static_assert(NB <= 32, "hit mask");
   const int64_t tid = threadIdx.x;
   int64_t a = s;
   const int64_t head_end = aligned_head_end(d.in, a, e);
-  for (int64_t i = a + tid; i < head_end; i += BLOCK) one(load_elem(d.in, i), (uint32_t)i);
+  if (a < head_end) one(a + tid < head_end, a + tid);
   a = head_end;
   const int64_t nvec = (e - a) / EPV;
   const RADIK_AS1 u32x4* __restrict__ pv =
       reinterpret_cast<const RADIK_AS1 u32x4*>(glb(reinterpret_cast<const char*>(d.in)) + a * EB);
-  int64_t v = tid;
-  for (; v + (int64_t)(UNR - 1) * BLOCK < nvec; v += (int64_t)UNR * BLOCK) {
+  constexpr int NB = EPV * UNR;  // elements per lane per iteration
+  int64_t vb = 0;                // first vector of the iteration (lane 0's)
+  if (nvec >= (int64_t)UNR * BLOCK) {
     u32x4 r[UNR];
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) r[u] = __builtin_nontemporal_load(pv + v + (int64_t)u * BLOCK);
-    uint32_t hit = 0;
-    const uint32_t p0 = (uint32_t)(a + v * EPV);  // position of this lane's first element (POS)
+    for (int u = 0; u < UNR; ++u) r[u] = __builtin_nontemporal_load(pv + vb + tid + (int64_t)u * BLOCK);
+    while (true) {
+      const int64_t vn = vb + (int64_t)UNR * BLOCK;
+      const bool more = vn + (int64_t)UNR * BLOCK <= nvec;
+      u32x4 rn[UNR];
+      if (more) {
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-#pragma unroll
-      for (int j = 0; j < EPV; ++j) {
-        const uint32_t rk = rank_bits<SCALE>(vec_elem(r[u], j), sc, xm);
-        atomicAdd(&lh[rk >> 20], 1u);
-        if (SPILL && spills(rk, p0 + (uint32_t)(u * BLOCK * EPV + j))) hit |= 1u << (EPV * u + j);
+        for (int u = 0; u < UNR; ++u) rn[u] = __builtin_nontemporal_load(pv + vn + tid + (int64_t)u * BLOCK);
       }
-    }
-    if (SPILL && ballot(hit != 0)) {
-      {
-        uint64_t mk[NB];
-        uint32_t total = 0;
+      uint32_t rk[NB];
+      uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          mk[b] = ballot((hit >> b) & 1u);
-          total += (uint32_t)__builtin_popcountll(mk[b]);
-        }
-        // The stage keeps stage_append's invariant: at most STAGE - 64 entries
-        // between appends.
-        lds_wave_sync();
-        uint32_t n0 = *st.n;
-        if (n0 + total > STAGE - 64) {
-          stage_flush(st, ctr, write);
-          n0 = 0;
-        }
-        uint32_t off;
-        const bool direct = total > STAGE - 64;  // more than a stage holds: straight to the spill
-        if (direct) {
-          uint32_t base = 0;
-          if (lanes_below(ballot(true)) == 0) base = atomicAdd(ctr, total);
-          off = __builtin_amdgcn_readfirstlane(base);
-        } else {
-          off = n0;
-        }
+      for (int u = 0; u < UNR; ++u) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          if ((hit >> b) & 1u) {
-            const int u = b / EPV, j = b % EPV;
-            const uint32_t pos = (uint32_t)(a + (v + (int64_t)u * BLOCK) * EPV) + (uint32_t)j;
-            const uint64_t ent = ((uint64_t)rank_bits<SCALE>(vec_elem(r[u], j), sc, xm) << 32) | pos;
-            const uint32_t slot = off + lanes_below(mk[b]);
-            if (direct) write(slot, ent);
-            else st.buf[slot] = ent;
-          }
-          off += (uint32_t)__builtin_popcountll(mk[b]);
-        }
-        if (!direct) {
-          lds_wave_sync();
-          if (lanes_below(ballot(true)) == 0) *st.n = n0 + total;
-          lds_wave_sync();
+        for (int j = 0; j < EPV; ++j) {
+          rk[EPV * u + j] = rank_bits<SCALE>(vec_elem(r[u], j), sc, xm);
+          atomicAdd(&lh[rk[EPV * u + j] >> 20], 1u);
+          if (SPILL) mn = min(mn, rk[EPV * u + j]);
         }
       }
+      if (SPILL && ballot(mn <= rp)) {
+        const uint32_t p0 = (uint32_t)(a + (vb + tid) * EPV);  // position of this lane's first element
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const uint32_t pos = p0 + (uint32_t)((b / EPV) * BLOCK * EPV + (b % EPV));
+          append(spills(rk[b], pos), rk[b], pos);
+        }
+      }
+      vb = vn;
+      if (!more) break;
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) r[u] = rn[u];
     }
   }
-  for (; v < nvec; v += BLOCK) {
-    const u32x4 r = __builtin_nontemporal_load(pv + v);
+  for (; vb < nvec; vb += BLOCK) {
+    const int64_t v = vb + tid;
+    const bool ok = v < nvec;
+    u32x4 r = {0u, 0u, 0u, 0u};
+    if (ok) r = __builtin_nontemporal_load(pv + v);
     const uint32_t q = (uint32_t)(a + v * EPV);
 #pragma unroll
-    for (int j = 0; j < EPV; ++j) one(vec_elem(r, j), q + (uint32_t)j);
+    for (int j = 0; j < EPV; ++j) {
+      uint32_t rk = 0;
+      if (ok) {
+        rk = rank_bits<SCALE>(vec_elem(r, j), sc, xm);
+        atomicAdd(&lh[rk >> 20], 1u);
+      }
+      if (SPILL) append(ok && spills(rk, q + (uint32_t)j), rk, q + (uint32_t)j);
+    }
   }
-  for (int64_t i = a + nvec * EPV + tid; i < e; i += BLOCK) one(load_elem(d.in, i), (uint32_t)i);
-  if (SPILL) stage_flush(st, ctr, write);
+  const int64_t t0 = a + nvec * EPV;  // fewer than EPV elements remain
+  if (t0 < e) one(t0 + tid < e, t0 + tid);
+  if (SPILL && n) flush();
 }
 
 // Relaxed device-scope load: reads where device-scope atomics are performed,
@@ -837,7 +852,7 @@ template <int UNR, bool SELECT, bool POSK>
 __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, const int64_t* __restrict__ bases,
                                            int ntasks, int64_t g0, int64_t g1, int t, uint32_t* __restrict__ hist,
                                            TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
-                                           uint32_t fin_cap, uint32_t* lh, const WaveStage& st, uint32_t* wsum,
+                                           uint32_t fin_cap, uint32_t* lh, uint64_t* stage, uint32_t* wsum,
                                            uint32_t* res, int& pend) {
   if (pend >= 0 && pend != t) {
     flush_hist(lh, hist + (size_t)pend * HIST_BINS);
@@ -856,14 +871,14 @@ __device__ __forceinline__ void hist_chunk(const TaskDesc* __restrict__ tasks, c
     uint64_t* spill = cand + d.base;
     const int64_t s = g0 - tb, e = se - tb;
     if (bp == NO_BIN) {
-      if (scale) pass_a_segment<UNR, true, false, false>(d, s, e, sc, xm, 0u, 0u, lh, ctr, spill, st);
-      else pass_a_segment<UNR, false, false, false>(d, s, e, sc, xm, 0u, 0u, lh, ctr, spill, st);
+      if (scale) pass_a_segment<UNR, true, false, false>(d, s, e, sc, xm, 0u, 0u, lh, ctr, spill, stage);
+      else pass_a_segment<UNR, false, false, false>(d, s, e, sc, xm, 0u, 0u, lh, ctr, spill, stage);
     } else if (!POSK || pp == 0xFFFFFFFFu) {
-      if (scale) pass_a_segment<UNR, true, true, false>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
-      else pass_a_segment<UNR, false, true, false>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
+      if (scale) pass_a_segment<UNR, true, true, false>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, stage);
+      else pass_a_segment<UNR, false, true, false>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, stage);
     } else {
-      if (scale) pass_a_segment<UNR, true, true, POSK>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
-      else pass_a_segment<UNR, false, true, POSK>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, st);
+      if (scale) pass_a_segment<UNR, true, true, POSK>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, stage);
+      else pass_a_segment<UNR, false, true, POSK>(d, s, e, sc, xm, rp, pp, lh, ctr, spill, stage);
     }
     __syncthreads();
     if (se >= g1 && se < tb + d.len) {
@@ -907,18 +922,14 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const TaskDesc* __restrict__ tas
                                                 const uint32_t* __restrict__ cmap, uint32_t fin_cap) {
   __shared__ uint32_t lh[HIST_BINS];
   __shared__ uint64_t stage_buf[BLOCK / 64][STAGE];
-  __shared__ uint32_t stage_n[BLOCK / 64];
   __shared__ uint32_t wsum[BLOCK / 64];
   __shared__ uint32_t res[3];
   for (int i = threadIdx.x; i < HIST_BINS; i += BLOCK) lh[i] = 0;
-  if (threadIdx.x < BLOCK / 64) stage_n[threadIdx.x] = 0;
-  const WaveStage st{stage_buf[threadIdx.x >> 6], &stage_n[threadIdx.x >> 6]};
   const int ch = blockIdx.x;
   __syncthreads();
   int pend = -1;
   hist_chunk<UNR, SELECT, POSK>(tasks, bases, ntasks, cstart[ch], cstart[ch + 1], (int)cmap[ch], hist, ctl, cand,
-                                fin_cap,
-                          lh, st, wsum, res, pend);
+                                fin_cap, lh, stage_buf[threadIdx.x >> 6], wsum, res, pend);
   if (pend >= 0) flush_hist(lh, hist + (size_t)pend * HIST_BINS);
 }
 
@@ -1590,6 +1601,16 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict_
 // Dynamic LDS: cap entries (v), then min(max m, cap) entries (sel) when fin.
 // ---------------------------------------------------------------------------
 constexpr int FIN_UNR = 8;  // float4 loads in flight per thread when re-reading a task
+// Minimum waves per SIMD the compiler keeps room for (register budget) in
+// k_finish (E <= 8) and k_medium.
+#ifndef RADIK_FIN_WAVES_PER_EU
+#define RADIK_FIN_WAVES_PER_EU 8
+#endif
+#ifndef RADIK_MED_WAVES_PER_EU
+#define RADIK_MED_WAVES_PER_EU 8
+#endif
+constexpr int FIN_WAVES_PER_EU = RADIK_FIN_WAVES_PER_EU;
+constexpr int MED_WAVES_PER_EU = RADIK_MED_WAVES_PER_EU;
 
 // LDS of a finishing block beside its dynamic entry array.
 struct FinShared {
@@ -1935,7 +1956,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
 }
 
 template <int EMAX>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8 ? 8 : 1))) void k_finish(const TaskDesc* __restrict__ tasks,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8 ? FIN_WAVES_PER_EU : 1))) void k_finish(const TaskDesc* __restrict__ tasks,
                                                   TaskCtl* __restrict__ ctl,
                                                   uint32_t* __restrict__ hist,
                                                   uint64_t* __restrict__ cand, uint32_t cap, uint32_t fin_cap,
@@ -2230,7 +2251,7 @@ __global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ 
 // here (m <= cap).
 // ---------------------------------------------------------------------------
 template <int EMAX>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_medium(const TaskDesc* __restrict__ tasks,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MED_WAVES_PER_EU))) void k_medium(const TaskDesc* __restrict__ tasks,
                                                   uint64_t* __restrict__ cand, uint32_t cap) {
   extern __shared__ uint64_t v[];
   __shared__ FinShared S;

@@ -17,6 +17,10 @@ namespace RADIK_KNS {
 // carry twice the elements per load, so their widest setting is 2.
 constexpr int UNR_MAX = (EB == 2) ? 2 : 4;
 
+#ifndef RADIK_MED_FULL_CAP
+#define RADIK_MED_FULL_CAP 1
+#endif
+
 // ---------------------------------------------------------------------------
 // Kernel sequence
 // ---------------------------------------------------------------------------
@@ -51,13 +55,21 @@ static int launch_short_and_medium(const Pipeline& p, const DevFacts& f, const L
   if (p.n_med) {
     // (at least 1/(MED_MAX_PER_CU + 1) of a CU's LDS per block, so a CU keeps
     // room for the sampling and prediction kernels beside the medium tasks)
-    const size_t lds = std::max<size_t>(((size_t)MED_CAP + std::min<uint32_t>(p.max_m_med, MED_CAP)) * 8,
-                                        f.lds_per_block / (MED_MAX_PER_CU + 1) + 256);
+    const uint32_t sel = std::min<uint32_t>(p.max_m_med, MED_CAP);
+    const size_t lds = std::max<size_t>(((size_t)MED_CAP + sel) * 8,
+                                        (f.lds_per_block / (MED_MAX_PER_CU + 1) + 256) / 8 * 8);
+    // every entry of the allocation not reserved for the m results holds
+    // winners and candidates (a task with more finishes in global memory)
+#if RADIK_MED_FULL_CAP
+    const uint32_t cap = (uint32_t)(lds / 8) - sel;
+#else
+    const uint32_t cap = MED_CAP;
+#endif
     const dim3 g((unsigned)p.n_med), b(BLOCK);
-    if (p.max_m_med <= (uint32_t)BLOCK) hipLaunchKernelGGL(k_medium<1>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
-    else if (p.max_m_med <= 2u * BLOCK) hipLaunchKernelGGL(k_medium<2>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
-    else if (p.max_m_med <= 4u * BLOCK) hipLaunchKernelGGL(k_medium<4>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
-    else hipLaunchKernelGGL(k_medium<8>, g, b, lds, s, p.d_med, p.d_cand, MED_CAP);
+    if (p.max_m_med <= (uint32_t)BLOCK) hipLaunchKernelGGL(k_medium<1>, g, b, lds, s, p.d_med, p.d_cand, cap);
+    else if (p.max_m_med <= 2u * BLOCK) hipLaunchKernelGGL(k_medium<2>, g, b, lds, s, p.d_med, p.d_cand, cap);
+    else if (p.max_m_med <= 4u * BLOCK) hipLaunchKernelGGL(k_medium<4>, g, b, lds, s, p.d_med, p.d_cand, cap);
+    else hipLaunchKernelGGL(k_medium<8>, g, b, lds, s, p.d_med, p.d_cand, cap);
     RADIK_HIP(hipGetLastError());
   }
   return 0;
@@ -77,12 +89,18 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
   // phase timing they run on the side stream, overlapping the sampling and
   // pass A.
   const bool fork = !timed && (p.n_small || p.n_med) && p.n_large && p.side;
-  if (fork) {
+  auto fork_side = [&]() -> int {
     RADIK_HIP(hipEventRecord(p.ev_fork, s));
     RADIK_HIP(hipStreamWaitEvent(p.side, p.ev_fork, 0));
-    if ((rc = launch_short_and_medium(p, f, o, p.side))) return rc;
+    int r = launch_short_and_medium(p, f, o, p.side);
+    if (r) return r;
     RADIK_HIP(hipEventRecord(p.ev_join, p.side));
-  }
+    return 0;
+  };
+  // fork_late: the side stream starts once the prediction is done, so the
+  // sampling and prediction have the whole chip and the short and medium
+  // tasks overlap pass A only
+  if (fork && !o.fork_late && (rc = fork_side())) return rc;
   if (p.n_large) {
     // k_finish's LDS: cap entries for a task's winners + candidates (a power
     // of two >= 2048, which also holds the 4096-bin histogram of the wide
@@ -120,6 +138,7 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
                          p.d_shist, p.d_ctl);
       RADIK_HIP(hipGetLastError());
       if ((rc = mark())) return rc;
+      if (fork && o.fork_late && (rc = fork_side())) return rc;
       switch (o.unr) {
         case 1: launch_stream_kernels_hist<1>(p, fin_cap, s); break;
         case 4: launch_stream_kernels_hist<UNR_MAX>(p, fin_cap, s); break;
