@@ -153,7 +153,9 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * elements run whole in one block beside the streaming passes, or join
  * the streaming passes and are re-read by the finishing kernel), and
  * "fork_late" (0/1: the short and medium tasks' kernels start with the
- * launch, or once the long tasks' prediction is done). */
+ * launch, or once the long tasks' prediction is done) and "medium_per_cu"
+ * (1..16: how many medium-task blocks may share a CU; each reserves that
+ * share of the CU's LDS). */
 int radik_plan_set_option(radik_plan *plan, const char *name, int64_t value);
 /* Per-kernel timing: when enabled, execute runs eagerly (every kernel on the
  * plan's stream) and records HIP events between the phases; radik_plan_timing

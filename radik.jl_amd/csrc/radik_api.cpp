@@ -445,6 +445,9 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
 
   } else if (n == "fork_late") {
     p->opts.fork_late = value != 0;
+  } else if (n == "medium_per_cu") {
+    if (value < 1 || value > 16) return set_error(RADIK_EINVAL, "medium_per_cu must be in 1..16");
+    p->opts.med_per_cu = (int)value;
   } else if (n == "predict") {
     p->opts.predict = value != 0;
   } else if (n == "blocks_per_cu") {
@@ -466,6 +469,7 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
     p->opts.predict = keep.predict;
     p->opts.medium = keep.medium;
     p->opts.fork_late = keep.fork_late;
+    p->opts.med_per_cu = keep.med_per_cu;
   } else {
     return set_error(RADIK_EINVAL, "unknown plan option '%s'", name);
   }

@@ -57,6 +57,7 @@ struct LaunchOpts {
   bool predict = true;      // sample long tasks and spill in pass A
   bool medium = true;       // medium tasks to k_medium (else the streaming passes)
   bool fork_late = false;   // short/medium tasks start after the prediction (else with the launch)
+  int med_per_cu = MED_MAX_PER_CU;  // k_medium blocks that may share a CU (LDS reserved per block)
   int key_type = KEY_F32;   // element type of every task of the launch (KEY_*)
 };
 inline uint32_t eff_small_cap(const DevFacts& f, const LaunchOpts& o) {

@@ -808,7 +808,6 @@ __device__ __forceinline__ void pass_a_segment(const TaskDesc& d, int64_t s, int
       }
       vb = vn;
       if (!more) break;
-#pragma unroll
       for (int u = 0; u < UNR; ++u) r[u] = rn[u];
     }
   }

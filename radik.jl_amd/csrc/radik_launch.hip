@@ -53,11 +53,11 @@ static int launch_short_and_medium(const Pipeline& p, const DevFacts& f, const L
     RADIK_HIP(hipGetLastError());
   }
   if (p.n_med) {
-    // (at least 1/(MED_MAX_PER_CU + 1) of a CU's LDS per block, so a CU keeps
+    // (at least 1/(med_per_cu + 1) of a CU's LDS per block, so a CU keeps
     // room for the sampling and prediction kernels beside the medium tasks)
     const uint32_t sel = std::min<uint32_t>(p.max_m_med, MED_CAP);
     const size_t lds = std::max<size_t>(((size_t)MED_CAP + sel) * 8,
-                                        (f.lds_per_block / (MED_MAX_PER_CU + 1) + 256) / 8 * 8);
+                                        (f.lds_per_block / (o.med_per_cu + 1) + 256) / 8 * 8);
     // every entry of the allocation not reserved for the m results holds
     // winners and candidates (a task with more finishes in global memory)
 #if RADIK_MED_FULL_CAP
