@@ -47,8 +47,10 @@ constexpr uint32_t SORT_MAX_RUNS = 64;
 // selection (rank < m) and its output slot.  At most RT_MAX_RUNS runs.
 constexpr uint32_t RT_RUN = BLOCK * 2;
 constexpr int64_t RT_MAX_RUNS = 256;
-// Entries of the runs a rank block holds in LDS at once.
+// Entries of the runs a rank block holds in LDS at once (the launch's
+// longest list up to RT_LDS, and at most RT_RANK_LDS).
 constexpr uint32_t RT_LDS = 16384;
+constexpr uint32_t RT_RANK_LDS = 8192;
 
 // One task of a launch.  Pointers address the task's own first element, so a
 // launch may mix tasks from different buffers (the serving engine does).

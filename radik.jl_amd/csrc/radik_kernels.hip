@@ -2088,9 +2088,10 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
 // behind the spill.  The m best of the task are the m smallest of L.
 //   k_rt_runs  one block per run of RT_RUN entries of L: sorts it in place
 //              (register bitonic)
-//   k_rt_rank  one block per BLOCK entries of L: the entry's rank r over L
-//              (its place in its own run plus a binary search in every other
-//              run, the runs staged through LDS) selects it (r < m) and gives
+//   k_rt_rank  one block per RT_ENTRIES entries of L: the entry's rank r
+//              over L (its place in its own run plus a binary search in
+//              every other run, the runs staged through LDS, the block's
+//              waves taking every fourth run each) selects it (r < m) and gives
 //              its output slot: r itself when the output order is the
 //              selection order, else the slot in the reversed order of rank
 //              keys with equal keys by position (their group bounds come from
@@ -2102,6 +2103,8 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
 // runs searched together.
 constexpr int RT_FILL = 16;
 constexpr int RT_ILP = 8;
+// List entries per rank block (one per lane; its waves split the runs).
+constexpr uint32_t RT_ENTRIES = 64;
 
 __device__ __forceinline__ uint32_t rt_list_len(const TaskCtl& c) {
   return c.nwin + (c.list_ctr < c.ncand ? c.list_ctr : c.ncand);
@@ -2148,19 +2151,24 @@ __global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ 
                                                    const TaskCtl* __restrict__ ctl, const uint64_t* __restrict__ cand,
                                                    const uint2* __restrict__ rmap, uint32_t lds_entries) {
   extern __shared__ uint64_t lr[];  // lds_entries, a multiple of RT_RUN
-  constexpr uint32_t HALVES = RT_RUN / BLOCK;
-  const uint2 tj = rmap[blockIdx.x / HALVES];
+  __shared__ uint32_t part[3][BLOCK / 64 - 1][RT_ENTRIES];  // waves 1..3: rank, glo, ghi
+  constexpr uint32_t PARTS = RT_RUN / RT_ENTRIES;
+  constexpr uint32_t WAVES = BLOCK / 64;
+  const uint2 tj = rmap[blockIdx.x / PARTS];
   const uint32_t t = tj.x, j = tj.y & 0xFFFFu;
   const TaskCtl c = ctl[t];
   if (!c.rt) return;
   const TaskDesc d = tasks[t];
   const uint32_t nl = rt_list_len(c), m = c.m;
-  const uint32_t q0 = j * RT_RUN + (blockIdx.x % HALVES) * BLOCK;
+  const uint32_t q0 = j * RT_RUN + (blockIdx.x % PARTS) * RT_ENTRIES;
   if (q0 == 0)  // the task's first block pads the slots past m
     for (uint32_t i = m + threadIdx.x; i < (uint32_t)d.k; i += BLOCK) write_slot(d, i, 0ull, false);
   if (q0 >= nl) return;
   const uint64_t* __restrict__ L = cand + d.base + c.cand_ctr;
-  const uint32_t qi = q0 + threadIdx.x;
+  // every wave holds the block's RT_ENTRIES entries, one per lane, and
+  // searches every WAVES-th run
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t qi = q0 + (threadIdx.x & 63u);
   const bool valid = qi < nl;
   const uint64_t x = valid ? L[qi] : ~0ull;
   const uint32_t own = qi / RT_RUN, runs = (nl + RT_RUN - 1) / RT_RUN;
@@ -2188,14 +2196,14 @@ __global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ 
       }
     }
     __syncthreads();
-    for (uint32_t r = r0; r < r1; r += RT_ILP) {
+    for (uint32_t r = r0 + w; r < r1; r += WAVES * RT_ILP) {
       // the binary searches of RT_ILP runs interleaved, step by step, so
       // their LDS reads are in flight together
       // (branch-free: a run past r1 has length 0 and probes entry 0)
       uint32_t lbs[RT_ILP], rls[RT_ILP], bas[RT_ILP];
 #pragma unroll
       for (int q = 0; q < RT_ILP; ++q) {
-        const uint32_t rr = r + (uint32_t)q;
+        const uint32_t rr = r + (uint32_t)q * WAVES;
         const bool ok = rr < r1;
         rls[q] = ok ? min(RT_RUN, nl - rr * RT_RUN) : 0u;
         bas[q] = ok ? (rr - r0) * RT_RUN : 0u;
@@ -2213,7 +2221,7 @@ __global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ 
       }
 #pragma unroll
       for (int q = 0; q < RT_ILP; ++q) {
-        const uint32_t rr = r + (uint32_t)q;
+        const uint32_t rr = r + (uint32_t)q * WAVES;
         if (rr >= r1) break;
         const uint64_t* run = lr + (rr - r0) * RT_RUN;
         const uint32_t rl = min(RT_RUN, nl - rr * RT_RUN);
@@ -2236,9 +2244,24 @@ __global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ 
       }
     }
   }
-  if (valid && rank < m) {
-    const uint32_t slot = same ? rank : m - min(m, ghi) + (rank - glo);
-    write_slot(d, slot, same ? x : out_of_selection(d, x), true);
+  // the waves' partial counts to wave 0
+  if (w > 0) {
+    part[0][w - 1][threadIdx.x & 63u] = rank;
+    part[1][w - 1][threadIdx.x & 63u] = glo;
+    part[2][w - 1][threadIdx.x & 63u] = ghi;
+  }
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (uint32_t v = 0; v + 1 < WAVES; ++v) {
+      rank += part[0][v][threadIdx.x];
+      glo += part[1][v][threadIdx.x];
+      ghi += part[2][v][threadIdx.x];
+    }
+    if (valid && rank < m) {
+      const uint32_t slot = same ? rank : m - min(m, ghi) + (rank - glo);
+      write_slot(d, slot, same ? x : out_of_selection(d, x), true);
+    }
   }
 }
 

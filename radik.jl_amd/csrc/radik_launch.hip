@@ -173,10 +173,13 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
       hipLaunchKernelGGL(k_rt_runs, dim3((unsigned)p.n_rt), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
                          p.d_rmap, p.d_shist);
       RADIK_HIP(hipGetLastError());
+      // (rank blocks hold at most RT_RANK_LDS entries of the list at a time,
+      // so two fit a CU)
       const uint32_t fit = (uint32_t)(f.lds_per_block / 8 / RT_RUN) * RT_RUN;
-      const uint32_t le = std::max<uint32_t>(RT_RUN, std::min<uint32_t>(p.rt_lds, fit));
-      hipLaunchKernelGGL(k_rt_rank, dim3((unsigned)(p.n_rt * (RT_RUN / BLOCK))), dim3(BLOCK), (size_t)le * 8, s,
-                         p.d_large, p.d_ctl, (const uint64_t*)p.d_cand, p.d_rmap, le);
+      const uint32_t le =
+          std::max<uint32_t>(RT_RUN, std::min<uint32_t>(std::min<uint32_t>(p.rt_lds, RT_RANK_LDS), fit));
+      hipLaunchKernelGGL(k_rt_rank, dim3((unsigned)(p.n_rt * (RT_RUN / RT_ENTRIES))), dim3(BLOCK), (size_t)le * 8,
+                         s, p.d_large, p.d_ctl, (const uint64_t*)p.d_cand, p.d_rmap, le);
       RADIK_HIP(hipGetLastError());
     }
     {
