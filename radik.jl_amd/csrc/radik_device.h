@@ -87,7 +87,7 @@ struct TaskCtl {
   uint32_t list_ctr;  // candidates listed by k_split
   uint32_t r_pred;    // pass A spills the composites <= (r_pred << 32 | p_pred)
   uint32_t p_pred;
-  uint32_t rt;        // 1: the rank tail (k_rt_runs, k_rt_rank) finishes the task
+  uint32_t rt;        // 1: the rank tail (run sorts in k_finish's launch, k_rt_rank) finishes the task
 };
 static_assert(sizeof(TaskCtl) == 64, "TaskCtl layout");
 

@@ -73,7 +73,7 @@ struct StageCounts {
   int64_t nch = 0;                 // pass A's chunks
   int64_t nbmap = 0;               // block-map entries (chunks + sampling blocks)
   int64_t nsplit = 0;              // k_split blocks
-  int64_t nrt = 0;                 // rank-tail runs (k_rt_runs blocks)
+  int64_t nrt = 0;                 // rank-tail runs (one run-sort block each)
 };
 
 // Device + host buffers of one pipeline instance.  Every array the host

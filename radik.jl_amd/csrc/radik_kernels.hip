@@ -985,8 +985,8 @@ __device__ __forceinline__ bool rt_cut(const TaskCtl& c) {
 // blocks count, for the rank-tail tasks with a cut (rt_cut), the second
 // digit (rank-key bits [19:8]) of their spilled candidates (first digit
 // b_cand) into the task's sample-histogram slot (free after the prediction;
-// k_rt_runs clears it): each block takes the part of the spill proportional
-// to its part of the task.  k_split cuts the candidates it lists at the
+// the rank tail's run sort clears it): each block takes the part of the
+// spill proportional to its part of the task.  k_split cuts the candidates it lists at the
 // second-digit bin holding the m-th best, so the rank tail sorts about m
 // entries instead of the whole candidate digit.  (The rank tail's list
 // before the cut fits its runs, so a task adds at most RT_MAX_RUNS * RT_RUN
@@ -1504,16 +1504,16 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const TaskDesc* __restrict__ tas
 // cbases: prefix sums of the runs per task; mbases: of the results per task.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restrict__ tasks,
-                                                       const TaskCtl* __restrict__ ctl,
-                                                       const int64_t* __restrict__ cbases, int ntasks,
-                                                       uint64_t* __restrict__ cand) {
-  __shared__ uint64_t xs[SORT_CHUNK];
-  const int t = find_task(cbases, ntasks, blockIdx.x);
+// Sort run b (xs: LDS scratch of SORT_CHUNK entries).
+__device__ __forceinline__ void sort_chunk_block(uint32_t b, const TaskDesc* __restrict__ tasks,
+                                                 const TaskCtl* __restrict__ ctl,
+                                                 const int64_t* __restrict__ cbases, int ntasks,
+                                                 uint64_t* __restrict__ cand, uint64_t* xs) {
+  const int t = find_task(cbases, ntasks, b);
   if (ctl[t].rt) return;  // sorted by the rank tail
   const TaskDesc d = tasks[t];
   const uint32_t m = ctl[t].m;
-  const uint32_t j0 = (uint32_t)(blockIdx.x - cbases[t]) * SORT_CHUNK;
+  const uint32_t j0 = (uint32_t)(b - cbases[t]) * SORT_CHUNK;
   const bool i64 = (d.flags & F_IDX64) != 0;
   uint64_t a[8];
 #pragma unroll
@@ -1536,6 +1536,14 @@ __global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restric
   }
   if (j0 == 0)  // padding slots past m
     for (uint32_t i = m + threadIdx.x; i < (uint32_t)d.k; i += BLOCK) write_slot(d, i, 0ull, false);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sort_chunks(const TaskDesc* __restrict__ tasks,
+                                                       const TaskCtl* __restrict__ ctl,
+                                                       const int64_t* __restrict__ cbases, int ntasks,
+                                                       uint64_t* __restrict__ cand) {
+  __shared__ uint64_t xs[SORT_CHUNK];
+  sort_chunk_block(blockIdx.x, tasks, ctl, cbases, ntasks, cand, xs);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_sort_rank(const TaskDesc* __restrict__ tasks,
@@ -1954,14 +1962,26 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   }
 }
 
+__device__ __forceinline__ void rt_runs_block(uint32_t b, const TaskDesc* __restrict__ tasks,
+                                              const TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
+                                              const uint2* __restrict__ rmap, uint32_t* __restrict__ shist,
+                                              uint64_t* xs);
+
+// Blocks [0, nfin): one per long task; blocks from nfin on: the rank tail's
+// run sorts (rmap, shist), which depend on the same kernels as the finishing.
 template <int EMAX>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8 ? FIN_WAVES_PER_EU : 1))) void k_finish(const TaskDesc* __restrict__ tasks,
                                                   TaskCtl* __restrict__ ctl,
                                                   uint32_t* __restrict__ hist,
                                                   uint64_t* __restrict__ cand, uint32_t cap, uint32_t fin_cap,
-                                                  int64_t big_len, int sort_here, int split) {
+                                                  int64_t big_len, int sort_here, int split, int nfin,
+                                                  const uint2* __restrict__ rmap, uint32_t* __restrict__ shist) {
   extern __shared__ uint64_t v[];
   __shared__ FinShared S;
+  if ((int)blockIdx.x >= nfin) {  // (v holds at least RT_RUN entries)
+    rt_runs_block(blockIdx.x - (uint32_t)nfin, tasks, ctl, cand, rmap, shist, v);
+    return;
+  }
   const int t = blockIdx.x;
   const TaskDesc d = tasks[t];
   TaskCtl c = ctl[t];
@@ -2086,8 +2106,8 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
 // (TaskCtl::rt, set at the selection): k_split has listed the task's winners
 // and spilled candidates as composites, list L = [nwin winners | candidates]
 // behind the spill.  The m best of the task are the m smallest of L.
-//   k_rt_runs  one block per run of RT_RUN entries of L: sorts it in place
-//              (register bitonic)
+//   runs       one block per run of RT_RUN entries of L (extra blocks of
+//              k_finish): sorts it in place (register bitonic)
 //   k_rt_rank  one block per RT_ENTRIES entries of L: the entry's rank r
 //              over L (its place in its own run plus a binary search in
 //              every other run, the runs staged through LDS, the block's
@@ -2102,7 +2122,7 @@ __global__ __launch_bounds__(BLOCK) void k_split(const TaskDesc* __restrict__ ta
 // Loads in flight per thread while a rank block fills its LDS with runs, and
 // runs searched together.
 constexpr int RT_FILL = 16;
-constexpr int RT_ILP = 8;
+constexpr int RT_ILP = 4;
 // List entries per rank block (one per lane; its waves split the runs).
 constexpr uint32_t RT_ENTRIES = 64;
 
@@ -2110,12 +2130,15 @@ __device__ __forceinline__ uint32_t rt_list_len(const TaskCtl& c) {
   return c.nwin + (c.list_ctr < c.ncand ? c.list_ctr : c.ncand);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_rt_runs(const TaskDesc* __restrict__ tasks,
-                                                   const TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
-                                                   const uint2* __restrict__ rmap, uint32_t* __restrict__ shist) {
+// Run b of the rank tail (rmap[b]: task, run index): sorts the run of the
+// task's list in place (xs: LDS scratch of RT_RUN entries).  Run by extra
+// blocks of k_finish, beside the finishing of the other tasks.
+__device__ __forceinline__ void rt_runs_block(uint32_t b, const TaskDesc* __restrict__ tasks,
+                                              const TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
+                                              const uint2* __restrict__ rmap, uint32_t* __restrict__ shist,
+                                              uint64_t* xs) {
   constexpr int E = (int)(RT_RUN / BLOCK);
-  __shared__ uint64_t xs[RT_RUN];
-  const uint2 tj = rmap[blockIdx.x];
+  const uint2 tj = rmap[b];
   const uint32_t t = tj.x, j = tj.y & 0xFFFFu;
   const TaskCtl c = ctl[t];
   if (!c.rt) return;
@@ -2147,20 +2170,26 @@ __device__ __forceinline__ uint32_t run_below(const uint64_t* run, uint32_t rl, 
   return pos;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ tasks,
-                                                   const TaskCtl* __restrict__ ctl, const uint64_t* __restrict__ cand,
-                                                   const uint2* __restrict__ rmap, uint32_t lds_entries) {
-  extern __shared__ uint64_t lr[];  // lds_entries, a multiple of RT_RUN
-  __shared__ uint32_t part[3][BLOCK / 64 - 1][RT_ENTRIES];  // waves 1..3: rank, glo, ghi
+// Partial counts of a rank block's waves 1..3 (rank, glo, ghi per entry).
+struct RankParts {
+  uint32_t v[3][BLOCK / 64 - 1][RT_ENTRIES];
+};
+
+// Rank block b (RT_RUN / RT_ENTRIES of them per run of rmap): lr holds
+// lds_entries of the list at a time (a multiple of RT_RUN).
+__device__ __forceinline__ void rt_rank_block(uint32_t b, const TaskDesc* __restrict__ tasks,
+                                              const TaskCtl* __restrict__ ctl, const uint64_t* __restrict__ cand,
+                                              const uint2* __restrict__ rmap, uint32_t lds_entries, uint64_t* lr,
+                                              RankParts& P) {
   constexpr uint32_t PARTS = RT_RUN / RT_ENTRIES;
   constexpr uint32_t WAVES = BLOCK / 64;
-  const uint2 tj = rmap[blockIdx.x / PARTS];
+  const uint2 tj = rmap[b / PARTS];
   const uint32_t t = tj.x, j = tj.y & 0xFFFFu;
   const TaskCtl c = ctl[t];
   if (!c.rt) return;
   const TaskDesc d = tasks[t];
   const uint32_t nl = rt_list_len(c), m = c.m;
-  const uint32_t q0 = j * RT_RUN + (blockIdx.x % PARTS) * RT_ENTRIES;
+  const uint32_t q0 = j * RT_RUN + (b % PARTS) * RT_ENTRIES;
   if (q0 == 0)  // the task's first block pads the slots past m
     for (uint32_t i = m + threadIdx.x; i < (uint32_t)d.k; i += BLOCK) write_slot(d, i, 0ull, false);
   if (q0 >= nl) return;
@@ -2246,23 +2275,44 @@ __global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ 
   }
   // the waves' partial counts to wave 0
   if (w > 0) {
-    part[0][w - 1][threadIdx.x & 63u] = rank;
-    part[1][w - 1][threadIdx.x & 63u] = glo;
-    part[2][w - 1][threadIdx.x & 63u] = ghi;
+    P.v[0][w - 1][threadIdx.x & 63u] = rank;
+    P.v[1][w - 1][threadIdx.x & 63u] = glo;
+    P.v[2][w - 1][threadIdx.x & 63u] = ghi;
   }
   __syncthreads();
   if (w == 0) {
 #pragma unroll
     for (uint32_t v = 0; v + 1 < WAVES; ++v) {
-      rank += part[0][v][threadIdx.x];
-      glo += part[1][v][threadIdx.x];
-      ghi += part[2][v][threadIdx.x];
+      rank += P.v[0][v][threadIdx.x];
+      glo += P.v[1][v][threadIdx.x];
+      ghi += P.v[2][v][threadIdx.x];
     }
     if (valid && rank < m) {
       const uint32_t slot = same ? rank : m - min(m, ghi) + (rank - glo);
       write_slot(d, slot, same ? x : out_of_selection(d, x), true);
     }
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rt_rank(const TaskDesc* __restrict__ tasks,
+                                                   const TaskCtl* __restrict__ ctl, const uint64_t* __restrict__ cand,
+                                                   const uint2* __restrict__ rmap, uint32_t lds_entries) {
+  extern __shared__ uint64_t lr[];  // lds_entries, a multiple of RT_RUN
+  __shared__ RankParts P;
+  rt_rank_block(blockIdx.x, tasks, ctl, cand, rmap, lds_entries, lr, P);
+}
+
+// k_rt_rank's blocks [0, nrank) and k_sort_chunks' after them in one launch
+// (both follow k_finish): lr holds max(lds_entries, SORT_CHUNK) entries.
+__global__ __launch_bounds__(BLOCK) void k_rank_sort(const TaskDesc* __restrict__ tasks,
+                                                     const TaskCtl* __restrict__ ctl, uint64_t* __restrict__ cand,
+                                                     const uint2* __restrict__ rmap, uint32_t lds_entries,
+                                                     uint32_t nrank, const int64_t* __restrict__ cbases,
+                                                     int ntasks) {
+  extern __shared__ uint64_t lr[];
+  __shared__ RankParts P;
+  if (blockIdx.x < nrank) rt_rank_block(blockIdx.x, tasks, ctl, cand, rmap, lds_entries, lr, P);
+  else sort_chunk_block(blockIdx.x - nrank, tasks, ctl, cbases, ntasks, cand, lr);
 }
 
 // ---------------------------------------------------------------------------

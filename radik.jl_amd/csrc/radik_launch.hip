@@ -168,42 +168,48 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
-    if (p.n_rt > 0) {
-      // rank tail of the tasks over BIG_LEN whose spill holds their m best
-      hipLaunchKernelGGL(k_rt_runs, dim3((unsigned)p.n_rt), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
-                         p.d_rmap, p.d_shist);
-      RADIK_HIP(hipGetLastError());
-      // (rank blocks hold at most RT_RANK_LDS entries of the list at a time,
-      // so two fit a CU)
-      const uint32_t fit = (uint32_t)(f.lds_per_block / 8 / RT_RUN) * RT_RUN;
-      const uint32_t le =
-          std::max<uint32_t>(RT_RUN, std::min<uint32_t>(std::min<uint32_t>(p.rt_lds, RT_RANK_LDS), fit));
-      hipLaunchKernelGGL(k_rt_rank, dim3((unsigned)(p.n_rt * (RT_RUN / RT_ENTRIES))), dim3(BLOCK), (size_t)le * 8,
-                         s, p.d_large, p.d_ctl, (const uint64_t*)p.d_cand, p.d_rmap, le);
-      RADIK_HIP(hipGetLastError());
-    }
     {
       const size_t lds = ((size_t)cap + sel) * 8;
-      const dim3 g((unsigned)p.n_large), b(BLOCK);
+      const dim3 b(BLOCK);
       const int sh = sort_here ? 1 : 0;
       const int sp = p.has_big ? 1 : 0;  // k_split listed the long tasks
+      // (extra blocks sort the rank tail's runs; k_finish's LDS holds a run)
+      const dim3 g((unsigned)(p.n_large + p.n_rt));
+      const int nf = (int)p.n_large;
       switch (emax) {
-        case 1: hipLaunchKernelGGL(k_finish<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 2: hipLaunchKernelGGL(k_finish<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 4: hipLaunchKernelGGL(k_finish<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 8: hipLaunchKernelGGL(k_finish<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 16: hipLaunchKernelGGL(k_finish<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        case 32: hipLaunchKernelGGL(k_finish<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
-        default: hipLaunchKernelGGL(k_finish<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp); break;
+        case 1: hipLaunchKernelGGL(k_finish<1>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp, nf, p.d_rmap, p.d_shist); break;
+        case 2: hipLaunchKernelGGL(k_finish<2>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp, nf, p.d_rmap, p.d_shist); break;
+        case 4: hipLaunchKernelGGL(k_finish<4>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp, nf, p.d_rmap, p.d_shist); break;
+        case 8: hipLaunchKernelGGL(k_finish<8>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp, nf, p.d_rmap, p.d_shist); break;
+        case 16: hipLaunchKernelGGL(k_finish<16>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp, nf, p.d_rmap, p.d_shist); break;
+        case 32: hipLaunchKernelGGL(k_finish<32>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp, nf, p.d_rmap, p.d_shist); break;
+        default: hipLaunchKernelGGL(k_finish<64>, g, b, lds, s, p.d_large, p.d_ctl, p.d_hist, p.d_cand, cap, fin_cap, BIG_LEN, sh, sp, nf, p.d_rmap, p.d_shist); break;
       }
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
+    // the rank tail's ranks (after its run sorts in k_finish's launch):
+    // beside k_sort_chunks in one launch when that sort runs
+    const uint32_t fit = (uint32_t)(f.lds_per_block / 8 / RT_RUN) * RT_RUN;
+    const uint32_t le =
+        std::max<uint32_t>(RT_RUN, std::min<uint32_t>(std::min<uint32_t>(p.rt_lds, RT_RANK_LDS), fit));
+    const unsigned nrank = (unsigned)(p.n_rt * (RT_RUN / RT_ENTRIES));
+    const bool chunk_sort = !sort_here && p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS;
+    if (p.n_rt > 0 && !chunk_sort) {
+      hipLaunchKernelGGL(k_rt_rank, dim3(nrank), dim3(BLOCK), (size_t)le * 8, s, p.d_large, p.d_ctl,
+                         (const uint64_t*)p.d_cand, p.d_rmap, le);
+      RADIK_HIP(hipGetLastError());
+    }
     if (!sort_here) {
-      if (p.max_m > 4096 && p.max_m <= SORT_CHUNK * SORT_MAX_RUNS) {
+      if (chunk_sort) {
         // several blocks per task: sorted runs, then ranks across runs
-        hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)p.total_runs), dim3(BLOCK), 0, s, p.d_large, p.d_ctl,
-                           p.d_cbases, (int)p.n_large, p.d_cand);
+        if (p.n_rt > 0)
+          hipLaunchKernelGGL(k_rank_sort, dim3(nrank + (unsigned)p.total_runs), dim3(BLOCK),
+                             (size_t)std::max<uint32_t>(le, SORT_CHUNK) * 8, s, p.d_large, p.d_ctl, p.d_cand,
+                             p.d_rmap, le, nrank, p.d_cbases, (int)p.n_large);
+        else
+          hipLaunchKernelGGL(k_sort_chunks, dim3((unsigned)p.total_runs), dim3(BLOCK), 0, s, p.d_large, p.d_ctl,
+                             p.d_cbases, (int)p.n_large, p.d_cand);
         RADIK_HIP(hipGetLastError());
         const unsigned rg = (unsigned)((p.total_m + BLOCK - 1) / BLOCK);
         hipLaunchKernelGGL(k_sort_rank, dim3(rg), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_mbases,
@@ -263,6 +269,7 @@ int RADIK_PASTE(launch_setup_, RADIK_KNS)(size_t lds_budget, int* hist_occ) {
               hipFuncSetAttribute((const void*)k_finish<32>, A, fin_want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_finish<64>, A, fin_want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_rt_rank, A, want) == hipSuccess &&
+              hipFuncSetAttribute((const void*)k_rank_sort, A, want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_sort<32>, A, want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_sort<64>, A, want) == hipSuccess &&
               hipFuncSetAttribute((const void*)k_small, A, want - 20 * 1024) == hipSuccess;
