@@ -444,7 +444,8 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
     p->opts.medium = value != 0;
 
   } else if (n == "fork_late") {
-    p->opts.fork_late = value != 0;
+    if (value < 0 || value > 2) return set_error(RADIK_EINVAL, "fork_late must be 0, 1 or 2");
+    p->opts.fork_late = (int)value;
   } else if (n == "medium_per_cu") {
     if (value < 1 || value > 16) return set_error(RADIK_EINVAL, "medium_per_cu must be in 1..16");
     p->opts.med_per_cu = (int)value;

@@ -56,7 +56,8 @@ struct LaunchOpts {
   uint32_t refine_cap = 0;  // candidates held in LDS by k_finish (0: device default)
   bool predict = true;      // sample long tasks and spill in pass A
   bool medium = true;       // medium tasks to k_medium (else the streaming passes)
-  bool fork_late = false;   // short/medium tasks start after the prediction (else with the launch)
+  int fork_late = 0;        // short/medium tasks start with the launch (0), after the prediction (1)
+                            // or after the sampling (2)
   int med_per_cu = MED_MAX_PER_CU;  // k_medium blocks that may share a CU (LDS reserved per block)
   int key_type = KEY_F32;   // element type of every task of the launch (KEY_*)
 };
