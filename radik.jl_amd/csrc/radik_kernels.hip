@@ -447,6 +447,23 @@ constexpr uint32_t PRED_MIN = 32;
 constexpr uint32_t PRED_CAP = 1024;
 static_assert(PRED_CAP == BLOCK * 4, "the prediction ranks PRED_CAP composites, 4 per thread");
 
+// Entry qq of the n <= BLOCK * E composites sv[0, n) in ascending order
+// (register bitonic, sv its scratch) into out.  Whole block.
+template <int E>
+__device__ __forceinline__ void refine_pick(uint64_t* sv, uint32_t n, uint32_t qq, uint64_t& out) {
+  uint64_t x[E];
+#pragma unroll
+  for (int l = 0; l < E; ++l) {
+    const uint32_t i = threadIdx.x * (uint32_t)E + (uint32_t)l;
+    x[l] = i < n ? sv[i] : ~0ull;
+  }
+  __syncthreads();  // sv is the sort's scratch from here
+  reg_bitonic<E>(x, sv);
+#pragma unroll
+  for (int l = 0; l < E; ++l)
+    if (threadIdx.x == qq / (uint32_t)E && (uint32_t)l == qq % (uint32_t)E) out = x[l];
+}
+
 // LDS of a prediction beside its PRED_CAP-entry array sv.
 struct PredShared {
   uint32_t wsum[BLOCK / 64];
@@ -504,49 +521,54 @@ __device__ __forceinline__ void predict_task(int t, const TaskDesc* __restrict__
       const float sc = task_scaler(d);
       const bool scale = task_scales(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
-      constexpr int U = 16;
+      constexpr int U = 32;  // groups in flight per round (a hit mask bit each)
       // group starts (one 64-bit division per group; G <= PRED_MAX_G)
       if (threadIdx.x < (uint32_t)G) P.gst[threadIdx.x] = (int64_t)threadIdx.x * d.len / G;
       __syncthreads();
       for (int64_t j0 = 0; j0 < G; j0 += U) {
         uint32_t xv[U];
-        uint32_t ps[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int64_t i = (j0 + u < G ? P.gst[j0 + u] : d.len) + threadIdx.x;
-          const bool ok = i < d.len;
-          xv[u] = ok ? load_elem(d.in, i) : 0u;
-          ps[u] = ok ? (uint32_t)i : 0xFFFFFFFFu;
+          xv[u] = i < d.len ? load_elem(d.in, i) : 0u;
         }
+        // this lane's hits as a mask; the wave reserves their slots with one
+        // atomic (a scan of the lanes' counts places them)
+        uint32_t hits = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+          const int64_t i = (j0 + u < G ? P.gst[j0 + u] : d.len) + threadIdx.x;
           const uint32_t rk = rank_key(xv[u], sc, xm, scale);
-          const bool hit = ps[u] != 0xFFFFFFFFu && (rk >> 20) == b;
-          const uint64_t hm = ballot(hit);
-          if (hm) {
-            const uint32_t slot = wave_append(hit, hm, &nsel);
-            if (hit && slot < PRED_CAP) sv[slot] = ((uint64_t)rk << 32) | ps[u];
+          if (i < d.len && (rk >> 20) == b) hits |= 1u << u;
+        }
+        const uint32_t nh = (uint32_t)__builtin_popcount(hits);
+        const uint32_t incl = wave_incl_scan(nh);
+        const uint32_t wtot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        uint32_t base = 0;
+        if (wtot) {
+          if (lane_id() == 0) base = atomicAdd(&nsel, wtot);
+          base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+        }
+        const uint32_t off = base + incl - nh;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if ((hits >> u) & 1u) {
+            const uint32_t slot = off + (uint32_t)__builtin_popcount(hits & ((1u << u) - 1u));
+            const uint32_t pos = (uint32_t)(P.gst[j0 + u] + threadIdx.x);
+            if (slot < PRED_CAP) sv[slot] = ((uint64_t)rank_key(xv[u], sc, xm, scale) << 32) | pos;
           }
         }
       }
       __syncthreads();
       if (nsel == cnt && need > before && need - before <= cnt) {
         // the q-th smallest sampled composite is the qi-th of digit b: sort
-        // the digit's sampled composites in registers (PRED_CAP = BLOCK * 4,
-        // +inf padding behind them) and take entry qi - 1
-        const uint32_t qi = need - before;
-        uint64_t x4[4];
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-          const uint32_t i = threadIdx.x * 4u + (uint32_t)l;
-          x4[l] = i < cnt ? sv[i] : ~0ull;
-        }
-        __syncthreads();  // sv is the sort's scratch from here
-        reg_bitonic<4>(x4, sv);
-        const uint32_t qq = qi - 1u;
-#pragma unroll
-        for (int l = 0; l < 4; ++l)
-          if (threadIdx.x == qq / 4u && (uint32_t)l == qq % 4u) P.qv = x4[l];
+        // the digit's sampled composites in registers (BLOCK * E entries for
+        // the smallest E that holds them, +inf padding behind) and take
+        // entry qi - 1
+        const uint32_t qq = need - before - 1u;
+        if (cnt <= (uint32_t)BLOCK) refine_pick<1>(sv, cnt, qq, P.qv);
+        else if (cnt <= 2u * BLOCK) refine_pick<2>(sv, cnt, qq, P.qv);
+        else refine_pick<4>(sv, cnt, qq, P.qv);
         __syncthreads();
         const uint64_t Ub = P.qv;
         rp = (uint32_t)(Ub >> 32);
