@@ -153,8 +153,11 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * elements run whole in one block beside the streaming passes, or join
  * the streaming passes and are re-read by the finishing kernel), and
  * "fork_late" (0/1/2: the short and medium tasks' kernels start with the
- * launch, once the long tasks' prediction is done, or once their sampling
- * is done) and "medium_per_cu"
+ * launch, once the long tasks' prediction is done, or -- the default --
+ * once their sampling is done), "sample_min_len" (shortest task that is
+ * sampled and gets a spill bound, default 32768; shorter ones run in
+ * k_medium) and
+ * "medium_per_cu"
  * (1..16: how many medium-task blocks may share a CU; each reserves that
  * share of the CU's LDS). */
 int radik_plan_set_option(radik_plan *plan, const char *name, int64_t value);

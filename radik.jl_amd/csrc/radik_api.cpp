@@ -446,6 +446,9 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
   } else if (n == "fork_late") {
     if (value < 0 || value > 2) return set_error(RADIK_EINVAL, "fork_late must be 0, 1 or 2");
     p->opts.fork_late = (int)value;
+  } else if (n == "sample_min_len") {
+    if (value < 4096 || value > ((int64_t)1 << 30)) return set_error(RADIK_EINVAL, "sample_min_len must be in 4096..2^30");
+    p->opts.sample_min_len = value;
   } else if (n == "medium_per_cu") {
     if (value < 1 || value > 16) return set_error(RADIK_EINVAL, "medium_per_cu must be in 1..16");
     p->opts.med_per_cu = (int)value;
@@ -471,6 +474,7 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
     p->opts.medium = keep.medium;
     p->opts.fork_late = keep.fork_late;
     p->opts.med_per_cu = keep.med_per_cu;
+    p->opts.sample_min_len = keep.sample_min_len;
   } else {
     return set_error(RADIK_EINVAL, "unknown plan option '%s'", name);
   }

@@ -254,13 +254,14 @@ int Pipeline::reserve(const StageCounts& c, int64_t tot, hipStream_t s) {
 }
 
 // Sample groups (of SAMPLE_GROUP elements) read to predict a long task's
-// first-digit bound: about 1/64 of the task, 16..1024 groups.  Tasks shorter
-// than 2^16 elements (re-reading them costs less than sampling), or whose k
-// exceeds 1/64 of their length, are not predicted: k_finish (or pass B)
-// re-reads them.
-static int64_t sample_groups(int64_t len, int64_t k) {
+// first-digit bound: about 1/64 of the task, 16..1024 groups (at least a
+// quarter of a task of 2^14 elements).  Tasks shorter than min_len elements
+// (default 2^15; re-reading them in k_medium costs less than sampling), or
+// whose k exceeds 1/64 of their length, are not predicted: k_medium,
+// k_finish (or pass B) re-reads them.
+static int64_t sample_groups(int64_t len, int64_t k, int64_t min_len) {
   const int64_t m = k < len ? k : len;
-  if (len < (int64_t)1 << 16 || m <= 0 || m * 64 > len) return 0;
+  if (len < min_len || len < (int64_t)SAMPLE_GROUP * 16 || m <= 0 || m * 64 > len) return 0;
   int64_t g = len / ((int64_t)SAMPLE_GROUP * 64);
   if (g < 16) g = 16;
   if (g > 1024) g = 1024;
@@ -369,7 +370,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   int64_t tot = 0, tot_med = 0, groups = 0;
   for (int64_t t = 0; t < n; ++t) {
     const int64_t len = tasks[t].len, k = tasks[t].k;
-    const int64_t g = o.predict ? sample_groups(len, k) : 0;
+    const int64_t g = o.predict ? sample_groups(len, k, o.sample_min_len) : 0;
     if (is_short(len, scap)) {
       ++cnt.ns;
     } else if (is_medium(len, k, g > 0, o)) {
@@ -399,7 +400,7 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   for (int64_t t = 0; t < n; ++t) {
     const TaskSpec& ts = tasks[t];
     const bool shrt = is_short(ts.len, scap);
-    const bool pred = o.predict && !shrt && sample_groups(ts.len, ts.k) > 0;
+    const bool pred = o.predict && !shrt && sample_groups(ts.len, ts.k, o.sample_min_len) > 0;
     const bool med = !shrt && is_medium(ts.len, ts.k, pred, o);
     if ((pass == 0) != pred) continue;
     TaskDesc d;
@@ -435,8 +436,8 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
       if (ts.len > BIG_LEN) big = true;
       p.h_large[il++] = d;
       base += ts.len;
-      groups += pred ? sample_groups(ts.len, ts.k) : 0;
-      if (pred && sample_groups(ts.len, ts.k) <= PRED_MAX_G) pos_bounds = true;
+      groups += pred ? sample_groups(ts.len, ts.k, o.sample_min_len) : 0;
+      if (pred && sample_groups(ts.len, ts.k, o.sample_min_len) <= PRED_MAX_G) pos_bounds = true;
     }
   }
   p.h_bases[il] = base;
@@ -515,7 +516,7 @@ void diag_route(const int64_t* lens, const int64_t* ks, int64_t n, uint32_t smal
   o.predict = predict != 0;
   o.medium = medium != 0;
   for (int64_t t = 0; t < n; ++t) {
-    const int64_t g = o.predict ? sample_groups(lens[t], ks[t]) : 0;
+    const int64_t g = o.predict ? sample_groups(lens[t], ks[t], o.sample_min_len) : 0;
     groups[t] = 0;
     if (is_short(lens[t], small_cap)) cls[t] = 0;
     else if (is_medium(lens[t], ks[t], g > 0, o)) cls[t] = 1;

@@ -66,6 +66,8 @@ def test_routing_rules():
         (0, 10, 0, 0), (2048, 100, 0, 0),           # short: run whole in one block
         (2049, 100, 1, 0), (65535, 2048, 1, 0),     # medium: one block beside the passes
         (65535, 2049, 3, 0),                        # m above the medium cap: streaming, unsampled
+        (32767, 256, 1, 0),                         # below the sampled length: medium
+        (32768, 256, 2, 16),                        # from 2^15 on sampled (at least 16 groups)
         (65536, 1024, 2, 16),                       # k <= len / 64: sampled (at least 16 groups)
         (65536, 1025, 1, 0),                        # not sampled -> medium
         (70000, 2000, 3, 0),                        # too long for medium, k > len / 64
