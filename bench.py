@@ -4,8 +4,8 @@ Default workload: the uneven batch (4096 tasks, lengths log-uniform in
 [2^10, 2^20], k = 256, per-task sigma log-uniform in [1e-3, 1e3], 30 %
 duplicates of 8 fixed values in a quarter of the tasks, a 1e-4 fraction of
 +-0 / +-min subnormal / +-Inf) through the persistent batched form (one plan,
-hipGraph replay).  One step = one top-k pass over the whole batch with the
-input resident in HBM.
+staged once, its kernels launched eagerly each step).  One step = one top-k
+pass over the whole batch with the input resident in HBM.
 
 The JSON line also carries, for the dominant kernel, its roofline
 (algorithmic bytes per launch / its mean event-timed duration) and the CPU
@@ -131,7 +131,7 @@ def time_plan(radik, plan, vo, io, d_x, steps, warmup):
 
 def phase_times(radik, plan, vo, io, d_x, steps, d_i=None):
     plan.set_timing(True)
-    for _ in range(2):  # the first eager executions after graph replays are slower
+    for _ in range(2):  # (the first phase-timed executions run slower)
         plan.execute(vo, io, d_x, d_i)
     radik.synchronize()
     plan.set_timing(True)  # (resets the sums)
@@ -512,10 +512,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded numpy, generated on host, resident in HBM before timing)",
         "config": {"workload": ("uneven batch: 4096 tasks, len log-uniform [2^10,2^20] (seed 4096), "
-                                f"k={k}, largest, rev=false, Int32 indices, persistent batched plan (hipGraph)"
+                                f"k={k}, largest, rev=false, Int32 indices, persistent batched plan (eager launches)"
                                 if args.workload == "uneven" else
                                 f"single task: 2^28 N(0,1) (seed 2828), k={k}, largest, rev=false, "
-                                "persistent batched plan (hipGraph)"),
+                                "persistent batched plan (eager launches)"),
                    "total_elements": N, "num_tasks": T, "k": k, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

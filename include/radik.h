@@ -128,14 +128,17 @@ int radik_topk_noalloc(float *val_out, void *idx_out, int idx_bytes, int64_t out
 /* Persistent batched form (no reference counterpart: the reference launches
  * one grid per pass, algorithm.jl:221-334).  A plan fixes task lengths, a k
  * per task and flags; radik_plan_execute enqueues the whole pass sequence
- * asynchronously, replaying a captured hipGraph.  Task t reads val_in at the
+ * asynchronously (eager launches, or the replay of a captured hipGraph
+ * after radik_plan_set_graph(plan, 1)).  Task t reads val_in at the
  * sum of earlier lengths and writes k_per_task[t] outputs to val_out/idx_out
  * at the sum of earlier k's (equal k gives the [k, T] layout). */
 int radik_plan_create(radik_plan **plan, const int64_t *task_lens, const int64_t *k_per_task,
                       int64_t num_tasks, uint32_t flags, int idx_bytes, void *stream);
 int radik_plan_execute(radik_plan *plan, void *val_out, void *idx_out, const void *val_in,
                        const void *idx_in, void *stream);
-/* use_graph = 0 launches the kernels eagerly (same results). */
+/* use_graph = 1 captures the sequence into a hipGraph and replays it, 0 (the
+ * default: faster on MI355X for every measured workload) launches the
+ * kernels eagerly; same results. */
 int radik_plan_set_graph(radik_plan *plan, int use_graph);
 /* Tuning knobs (results do not depend on them): vload_bytes 16/32/64 (bytes
  * in flight per lane per step of the streaming passes, 0 = keep), streaming
@@ -153,8 +156,8 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * elements run whole in one block beside the streaming passes, or join
  * the streaming passes and are re-read by the finishing kernel), and
  * "fork_late" (0/1/2: the short and medium tasks' kernels start with the
- * launch, once the long tasks' prediction is done, or -- the default --
- * once their sampling is done), "sample_min_len" (shortest task that is
+ * launch, once the long tasks' prediction is done, or once their sampling is
+ * done; -1, the default: 0 for eager launches, 2 for a graph), "sample_min_len" (shortest task that is
  * sampled and gets a spill bound, default 32768; shorter ones run in
  * k_medium) and
  * "medium_per_cu"

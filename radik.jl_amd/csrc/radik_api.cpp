@@ -65,7 +65,7 @@ struct radik_plan {
   void* last_idx_out = nullptr;
   bool staged = false;
   // graph
-  bool use_graph = true;
+  bool use_graph = false;  // eager launches (measured faster than graph replay on MI355X)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -444,7 +444,7 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
     p->opts.medium = value != 0;
 
   } else if (n == "fork_late") {
-    if (value < 0 || value > 2) return set_error(RADIK_EINVAL, "fork_late must be 0, 1 or 2");
+    if (value < -1 || value > 2) return set_error(RADIK_EINVAL, "fork_late must be -1, 0, 1 or 2");
     p->opts.fork_late = (int)value;
   } else if (n == "sample_min_len") {
     if (value < 4096 || value > ((int64_t)1 << 30)) return set_error(RADIK_EINVAL, "sample_min_len must be in 4096..2^30");
@@ -566,7 +566,9 @@ int radik_plan_execute(radik_plan* p, void* val_out, void* idx_out, const void* 
     }
     if (own) RADIK_HIP(hipStreamSynchronize(nullptr));
     RADIK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    rc = enqueue(p->pipe, *f, p->opts, cs, nullptr);
+    LaunchOpts go = p->opts;
+    go.graph = true;
+    rc = enqueue(p->pipe, *f, go, cs, nullptr);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(cs, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); if (own) (void)hipStreamDestroy(cs); return rc; }

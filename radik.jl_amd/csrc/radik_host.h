@@ -56,8 +56,9 @@ struct LaunchOpts {
   uint32_t refine_cap = 0;  // candidates held in LDS by k_finish (0: device default)
   bool predict = true;      // sample long tasks and spill in pass A
   bool medium = true;       // medium tasks to k_medium (else the streaming passes)
-  int fork_late = 2;        // short/medium tasks start with the launch (0), after the prediction (1)
-                            // or after the sampling (2)
+  int fork_late = -1;       // short/medium tasks start with the launch (0), after the prediction (1)
+                            // or after the sampling (2); -1: 0 for eager launches, 2 for a graph
+  bool graph = false;       // the sequence is being captured into a graph
   int med_per_cu = MED_MAX_PER_CU;  // k_medium blocks that may share a CU (LDS reserved per block)
   int64_t sample_min_len = (int64_t)1 << 15;  // shortest task that is sampled and predicted
   int key_type = KEY_F32;   // element type of every task of the launch (KEY_*)

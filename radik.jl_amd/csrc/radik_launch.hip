@@ -99,8 +99,11 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
   };
   // fork_late: the side stream starts once the prediction is done (1), so
   // the sampling and prediction have the whole chip and the short and medium
-  // tasks overlap pass A only, or once the sampling is done (2)
-  if (fork && o.fork_late == 0 && (rc = fork_side())) return rc;
+  // tasks overlap pass A only, or once the sampling is done (2).  By default
+  // eager launches fork at once and a captured graph after the sampling (its
+  // cross-stream edges cost more; measured best for each).
+  const int fl = o.fork_late >= 0 ? o.fork_late : (o.graph ? 2 : 0);
+  if (fork && fl == 0 && (rc = fork_side())) return rc;
   if (p.n_large) {
     // k_finish's LDS: cap entries for a task's winners + candidates (a power
     // of two >= 2048, which also holds the 4096-bin histogram of the wide
@@ -134,12 +137,12 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
         RADIK_HIP(hipGetLastError());
       }
       if ((rc = mark())) return rc;
-      if (fork && o.fork_late == 2 && (rc = fork_side())) return rc;
+      if (fork && fl == 2 && (rc = fork_side())) return rc;
       hipLaunchKernelGGL(k_predict, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_sbases,
                          p.d_shist, p.d_ctl);
       RADIK_HIP(hipGetLastError());
       if ((rc = mark())) return rc;
-      if (fork && o.fork_late == 1 && (rc = fork_side())) return rc;
+      if (fork && fl == 1 && (rc = fork_side())) return rc;
       switch (o.unr) {
         case 1: launch_stream_kernels_hist<1>(p, fin_cap, s); break;
         case 4: launch_stream_kernels_hist<UNR_MAX>(p, fin_cap, s); break;

@@ -1,8 +1,9 @@
 """Persistent batched form and serving engine over libradik_hip.so.
 
 Plan: one pass sequence for many tasks of uneven length, each with its own k,
-replayed from a captured hipGraph (no reference counterpart: the reference
-launches one grid per pass, RadiK.jl src/algorithm.jl:221-334).
+staged once and launched eagerly on every execute, or replayed from a
+captured hipGraph with use_graph=True (no reference counterpart: the
+reference launches one grid per pass, RadiK.jl src/algorithm.jl:221-334).
 Server: continuous batching of top-k requests against a device budget.
 """
 import ctypes
@@ -15,7 +16,7 @@ from .device import DeviceArray
 
 class Plan:
     def __init__(self, task_lens, k, largest=True, rev=False, withscale=False, indices_dtype=None,
-                 idx_dtype=np.int32, use_graph=True, stream=None, tuned=True, key="f32"):
+                 idx_dtype=np.int32, use_graph=False, stream=None, tuned=True, key="f32"):
         lens = [int(x) for x in task_lens]
         ks = [int(k)] * len(lens) if np.isscalar(k) else [int(x) for x in k]
         if len(ks) != len(lens):
