@@ -283,7 +283,7 @@ def tune_all(radik, out_path):
         p = radik.Plan(ll, k, tuned=False)
         vo = DeviceArray.zeros((k, T), np.float32)
         io = DeviceArray.zeros((k, T), np.int32)
-        best, ms, trials = autotune.tune(p, vo, io, d, steps=5, log=log)
+        best, ms, trials = autotune.tune(p, vo, io, d, steps=10, log=log)
         autotune.record(table, p, best, ms)
         log(f"tuned {name}: {best} {ms:.4f} ms (default {trials[0][1]:.4f} ms)")
         del d
