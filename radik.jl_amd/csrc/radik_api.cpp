@@ -446,6 +446,9 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
   } else if (n == "fork_late") {
     if (value < -1 || value > 2) return set_error(RADIK_EINVAL, "fork_late must be -1, 0, 1 or 2");
     p->opts.fork_late = (int)value;
+  } else if (n == "join_early") {
+    if (value < -1 || value > 1) return set_error(RADIK_EINVAL, "join_early must be -1, 0 or 1");
+    p->opts.join_early = (int)value;
   } else if (n == "sample_min_len") {
     if (value < 4096 || value > ((int64_t)1 << 30)) return set_error(RADIK_EINVAL, "sample_min_len must be in 4096..2^30");
     p->opts.sample_min_len = value;
@@ -473,6 +476,7 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
     p->opts.predict = keep.predict;
     p->opts.medium = keep.medium;
     p->opts.fork_late = keep.fork_late;
+    p->opts.join_early = keep.join_early;
     p->opts.med_per_cu = keep.med_per_cu;
     p->opts.sample_min_len = keep.sample_min_len;
   } else {

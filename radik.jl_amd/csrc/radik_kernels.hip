@@ -2007,13 +2007,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
   const int t = blockIdx.x;
   const TaskDesc d = tasks[t];
   TaskCtl c = ctl[t];
-  // The histogram is read before `done` is known (one latency instead of
-  // two); it is all zero for tasks pass A selected, and left so.
-  uint32_t* __restrict__ th = hist + (size_t)t * HIST_BINS;
-  uint32_t h[16];
-  load_hist(th, h);
   if (!c.done) {  // spread over several pass-A blocks: select from the global histogram
-    clear_hist(th);
+    // (read only here: the histograms of the tasks pass A selected are all
+    // zero, and reading them in every block costs the launch a burst of
+    // 16 KiB per task)
+    uint32_t h[16];
+    take_hist(hist + (size_t)t * HIST_BINS, h);
     select_first(d, h, c.b_pred, c.cand_ctr, fin_cap, &S.ctl, S.wsum, S.res);
     __syncthreads();
     c = S.ctl;

@@ -104,6 +104,8 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
   // cross-stream edges cost more; measured best for each).
   const int fl = o.fork_late >= 0 ? o.fork_late : (o.graph ? 2 : 0);
   if (fork && fl == 0 && (rc = fork_side())) return rc;
+  const bool join_here =
+      o.join_early >= 0 ? o.join_early != 0 : p.total >= p.total_med + p.total_small;  // (before k_finish)
   if (p.n_large) {
     // k_finish's LDS: cap entries for a task's winners + candidates (a power
     // of two >= 2048, which also holds the 4096-bin histogram of the wide
@@ -172,6 +174,10 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
       RADIK_HIP(hipGetLastError());
     }
     if ((rc = mark())) return rc;
+    // The side stream is waited for here, not at the end, when its tasks are
+    // expected to end before pass A: the wait is then resolved while pass A
+    // runs, and the next launch starts right behind k_finish.
+    if (fork && join_here) RADIK_HIP(hipStreamWaitEvent(s, p.ev_join, 0));
     {
       const size_t lds = ((size_t)cap + sel) * 8;
       const dim3 b(BLOCK);
@@ -241,7 +247,7 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
   }
   if ((rc = mark())) return rc;  // end of sort
   if (fork) {
-    RADIK_HIP(hipStreamWaitEvent(s, p.ev_join, 0));
+    if (!join_here) RADIK_HIP(hipStreamWaitEvent(s, p.ev_join, 0));
   } else if ((rc = launch_short_and_medium(p, f, o, s))) {
     return rc;
   }
