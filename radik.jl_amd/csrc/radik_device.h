@@ -84,7 +84,7 @@ struct TaskCtl {
   uint32_t spill_all; // 1: the spill holds the m best; 0: the task is re-read
   uint32_t fin;       // 1: the m best are gathered in k_finish's LDS (their source fits it)
   uint32_t done;      // 1: the first digit is selected
-  uint32_t list_ctr;  // candidates listed by k_split
+  uint32_t list_ctr;  // candidates listed by k_split, or by pass B over several blocks
   uint32_t r_pred;    // pass A spills the composites <= (r_pred << 32 | p_pred)
   uint32_t p_pred;
   uint32_t rt;        // 1: the rank tail (run sorts in k_finish's launch, k_rt_rank) finishes the task

@@ -712,9 +712,26 @@ __device__ __forceinline__ void clear_hist(uint32_t* __restrict__ h) {
   for (int q = 0; q < 4; ++q) h4[q] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// The selection fields of o into a task's control block, leaving the
+// counters win_ctr and list_ctr (which other blocks may be adding to).
+__device__ __forceinline__ void publish_selection(TaskCtl* g, const TaskCtl& o) {
+  g->b_win = o.b_win;
+  g->b_cand = o.b_cand;
+  g->krem = o.krem;
+  g->ncand = o.ncand;
+  g->nwin = o.nwin;
+  g->m = o.m;
+  g->cand_ctr = o.cand_ctr;
+  g->spill_all = o.spill_all;
+  g->fin = o.fin;
+  g->done = o.done;
+  g->rt = o.rt;
+}
+
 // Standalone selection, one block per task, of the tasks pass A did not
 // select (those spread over several of its blocks): reads and clears the
-// task's histogram.
+// task's histogram.  (The pass-level diagnostics run it; a launch selects
+// these tasks in k_compact or k_finish.)
 __global__ __launch_bounds__(BLOCK) void k_select(const TaskDesc* __restrict__ tasks,
                                                   uint32_t* __restrict__ hist,
                                                   TaskCtl* __restrict__ ctl, uint32_t fin_cap) {
@@ -1013,6 +1030,12 @@ __device__ __forceinline__ bool rt_cut(const TaskCtl& c) {
 // entries instead of the whole candidate digit.  (The rank tail's list
 // before the cut fits its runs, so a task adds at most RT_MAX_RUNS * RT_RUN
 // counts, merged per wave.)
+// Tasks longer than min_len (always spread over several blocks) are selected
+// here, by every block of the task from its global histogram (the same
+// result in each); the block holding the task's first element publishes it
+// (publish_selection), read by the kernels after this one.  Pass B counts its
+// winners in win_ctr and its candidates in list_ctr, which the selection
+// leaves alone (zero since the prediction).
 template <int UNR>
 __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ tasks,
                                                    const int64_t* __restrict__ bases, int ntasks,
@@ -1020,10 +1043,14 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
                                                    TaskCtl* __restrict__ ctl,
                                                    uint64_t* __restrict__ cand,
                                                    const uint32_t* __restrict__ cmap, int64_t min_len,
-                                                   uint32_t* __restrict__ shist) {
+                                                   uint32_t* __restrict__ shist, const uint32_t* __restrict__ hist,
+                                                   uint32_t fin_cap) {
   __shared__ uint64_t stage_w[BLOCK / 64][STAGE];
   __shared__ uint64_t stage_c[BLOCK / 64][STAGE];
   __shared__ uint32_t stage_n[2][BLOCK / 64];
+  __shared__ TaskCtl sel;
+  __shared__ uint32_t wsum[BLOCK / 64];
+  __shared__ uint32_t res[3];
   if (threadIdx.x < 2 * (BLOCK / 64)) stage_n[threadIdx.x / (BLOCK / 64)][threadIdx.x % (BLOCK / 64)] = 0;
   const WaveStage sw{stage_w[threadIdx.x >> 6], &stage_n[0][threadIdx.x >> 6]};
   const WaveStage scd{stage_c[threadIdx.x >> 6], &stage_n[1][threadIdx.x >> 6]};
@@ -1037,12 +1064,21 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const TaskDesc* __restrict__ 
     const TaskDesc d = tasks[t];
     const int64_t tb = bases[t];
     const int64_t se = min(g1, tb + d.len);
-    const TaskCtl c = ctl[t];
+    TaskCtl c = ctl[t];
+    if (d.len > min_len && !c.done) {
+      uint32_t h[16];
+      load_hist(hist + (size_t)t * HIST_BINS, h);
+      select_first(d, h, c.b_pred, c.cand_ctr, fin_cap, &sel, wsum, res);
+      __syncthreads();
+      c = sel;
+      if (g0 == tb && threadIdx.x == 0) publish_selection(&ctl[t], c);
+      __syncthreads();  // (sel is rewritten by the next task's selection)
+    }
     if (!c.spill_all && c.m > 0 && d.len > min_len) {
       const float sc = task_scaler(d);
       const uint32_t xm = (d.flags & F_LARGEST) ? 0xFFFFFFFFu : 0u;
       uint32_t* wctr = &ctl[t].win_ctr;
-      uint32_t* cctr = &ctl[t].cand_ctr;
+      uint32_t* cctr = &ctl[t].list_ctr;
       uint64_t* cb = cand + d.base;
       if (task_scales(d)) pass_b_segment<UNR, true>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
       else pass_b_segment<UNR, false>(d, g0 - tb, se - tb, sc, xm, c, wctr, cctr, cb, sw, scd);
@@ -1682,7 +1718,7 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   const bool fin = c.fin != 0;
   const uint32_t len = (uint32_t)d.len;
   uint64_t* __restrict__ region = cand + d.base;
-  const uint32_t n_sp = c.cand_ctr < len ? c.cand_ctr : len;  // spill, or pass B's candidates
+  const uint32_t n_sp = c.cand_ctr < len ? c.cand_ctr : len;  // the spill
   const bool big = d.len > big_len;
   // Where the winners and candidates are:
   //   listed  - pass B over several blocks (a long task re-read) or k_split
@@ -1694,11 +1730,12 @@ __device__ __forceinline__ void finish_task(const TaskDesc& d, const TaskCtl& c,
   const bool listed = reread_done || (split && big && split_fits(c, d.len));
   const uint64_t* __restrict__ lst = region + (reread_done ? 0u : n_sp);
   // (a spill need not hold the whole candidate bin: k_split lists the
-  // candidates it holds, list_ctr of them)
+  // candidates it holds, list_ctr of them; pass B counts its candidates in
+  // list_ctr too)
   const uint32_t nlisted = c.list_ctr < ncand ? c.list_ctr : ncand;
   // Pass B and k_split both list fin tasks' winners first (nwin of them),
-  // then the candidates; pass B counts its candidates in cand_ctr (n_sp).
-  const uint32_t nlst = (fin ? nwin : 0u) + (reread_done ? (n_sp < ncand ? n_sp : ncand) : nlisted);
+  // then the candidates.
+  const uint32_t nlst = (fin ? nwin : 0u) + nlisted;
   const bool spilled = c.spill_all && !listed;
   // Candidates of non-fin tasks go to LDS when they fit, else (a re-read
   // here) to the start of the spill region.  ncl: at most that many.
@@ -2017,6 +2054,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(EMAX <= 8
     __syncthreads();
     c = S.ctl;
     if (threadIdx.x == 0) ctl[t] = c;  // k_sort reads m
+  } else if (split && d.len > big_len) {
+    clear_hist(hist + (size_t)t * HIST_BINS);  // (selected in k_compact, which only reads it)
   }
   if (c.rt) return;  // finished by the rank tail
   finish_task<EMAX>(d, c, cand, v, cap, big_len, sort_here, split != 0, S);

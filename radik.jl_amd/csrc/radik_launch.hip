@@ -39,10 +39,11 @@ static void launch_stream_kernels_hist(const Pipeline& p, uint32_t fin_cap, hipS
 // Pass B over the whole element space (pass A's chunks, one per block);
 // blocks skip tasks of at most BIG_LEN elements and those with a usable spill.
 template <int UNR>
-static void launch_stream_kernels_compact(const Pipeline& p, hipStream_t s) {
+static void launch_stream_kernels_compact(const Pipeline& p, uint32_t fin_cap, hipStream_t s) {
   if (p.n_chunks <= 0) return;
   hipLaunchKernelGGL(k_compact<UNR>, dim3((unsigned)p.n_chunks), dim3(BLOCK), 0, s, p.d_large, p.d_bases,
-                     (int)p.n_large, p.d_cstart, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN, p.d_shist);
+                     (int)p.n_large, p.d_cstart, p.d_ctl, p.d_cand, p.d_bmap + p.off_h, BIG_LEN, p.d_shist,
+                     (const uint32_t*)p.d_hist, fin_cap);
 }
 
 // k_small and k_medium (independent of the long tasks).
@@ -153,20 +154,16 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
     }
     RADIK_HIP(hipGetLastError());
     if ((rc = mark())) return rc;
-    // Tasks spread over several pass-A blocks are selected by k_finish, or
-    // before pass B over several blocks (needed only for long tasks without a
-    // usable spill; k_finish re-reads the shorter ones).
-    if (p.has_big) {
-      hipLaunchKernelGGL(k_select, dim3((unsigned)p.n_large), dim3(BLOCK), 0, s, p.d_large, p.d_hist, p.d_ctl,
-                         fin_cap);
-      RADIK_HIP(hipGetLastError());
-    }
+    // Tasks spread over several pass-A blocks are selected by k_finish, and
+    // those longer than BIG_LEN by k_compact's blocks (pass B over several
+    // blocks re-reads the ones without a usable spill; k_finish re-reads the
+    // shorter ones).
     if ((rc = mark())) return rc;
     if (p.has_big) {
       switch (o.unr) {
-        case 1: launch_stream_kernels_compact<1>(p, s); break;
-        case 4: launch_stream_kernels_compact<UNR_MAX>(p, s); break;
-        default: launch_stream_kernels_compact<2>(p, s); break;
+        case 1: launch_stream_kernels_compact<1>(p, fin_cap, s); break;
+        case 4: launch_stream_kernels_compact<UNR_MAX>(p, fin_cap, s); break;
+        default: launch_stream_kernels_compact<2>(p, fin_cap, s); break;
       }
       RADIK_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_split, dim3((unsigned)p.n_split), dim3(BLOCK), 0, s, p.d_large, p.d_ctl, p.d_cand,
