@@ -7,7 +7,9 @@ position).
   graph-captured plan (short and medium tasks on the forked side stream) and
   eagerly;
 * a small plan mixing short, medium and long tasks, so the side-stream fork
-  inside a captured graph runs in every test run;
+  inside a captured graph runs in every test run, and the same kind of plan
+  under every launch knob (where the side stream forks and joins, the medium
+  tasks' LDS share, the sampling threshold), whose results must not change;
 * the 2^28-element task with Int32 custom indices (a seeded permutation),
   k in {16, 1024, 4096, 16384} x largest x rev (the grid of the reference's
   test/test_algorithm.jl:6-40 at the benchmark size);
@@ -92,6 +94,40 @@ def test_mixed_plan_side_stream():
                 same(vo.to_host(), io.to_host(), ev.T, ei.T, f"mixed k{k} graph={use_graph} rep{rep}")
             info = p.info()
             assert info["n_short"] > 0 and info["n_medium"] > 0 and info["n_long"] > 0, info
+
+
+def test_mixed_plan_launch_options():
+    # The plan's launch knobs change the schedule, never the results: the
+    # side stream joined before k_finish or at the end, forked at the start,
+    # after the sampling or after the prediction, other LDS shares for the
+    # medium tasks, other sampling thresholds; eager and graph replay.
+    rng = np.random.default_rng(78)
+    lens = [700, 2500, 40000, 65536, 1 << 18, 1 << 20, 3000, 33000, 0, 150000]
+    x = rng.standard_normal(sum(lens)).astype(np.float32)
+    x[rng.random(x.size) < 0.01] = np.float32(1.5)
+    d = DeviceArray.from_host(x)
+    k = 128
+    ev, ei = O.topk_exact(x, lens, k, O.LARGEST | O.ASCEND)
+    sets = [{"join_early": 0}, {"join_early": 1}, {"fork_late": 1}, {"fork_late": 2, "join_early": 0},
+            {"medium_per_cu": 1}, {"medium_per_cu": 8}, {"sample_min_len": 4096}, {"sample_min_len": 65536}]
+    for opts in sets:
+        for use_graph in (False, True):
+            say("mixed plan options", opts, "graph", use_graph)
+            p = radik.Plan(lens, k, largest=True, rev=False, use_graph=use_graph)
+            for name, val in opts.items():
+                p.set_option(name, val)
+            vo = DeviceArray.zeros((k, len(lens)), np.float32)
+            io = DeviceArray.zeros((k, len(lens)), np.int32)
+            for rep in range(2):
+                vo.fill(np.float32(-7))
+                p.execute(vo, io, d)
+                radik.synchronize()
+                same(vo.to_host(), io.to_host(), ev.T, ei.T, f"options {opts} graph={use_graph} rep{rep}")
+    p = radik.Plan(lens, k, largest=True, rev=False)
+    for name, bad in (("join_early", 2), ("fork_late", 3), ("medium_per_cu", 0), ("sample_min_len", 100),
+                      ("no_such_option", 1)):
+        with pytest.raises(radik.RadiKError):
+            p.set_option(name, bad)
 
 
 # ----------------------------------------------------------------- 2^28 task
