@@ -171,9 +171,11 @@ int radik_plan_set_option(radik_plan *plan, const char *name, int64_t value);
  * plan's stream) and records HIP events between the phases; radik_plan_timing
  * returns the accumulated milliseconds of the 8 phases (sample, predict,
  * hist = pass A with the selection of tasks inside one of its blocks, select
- * and compact = the selection and re-read of tasks over 2^20 elements when
- * present, finish, sort = only for results beyond the finishing kernel's
- * LDS, small = short and medium tasks) and the number of executes timed. */
+ * = no kernel of its own (the phase keeps its slot; only the gap between two
+ * event records), compact = the selection, re-read and listing of tasks over
+ * 2^20 elements when present, finish, sort = only for results beyond the
+ * finishing kernel's LDS and the rank tail's ranks, small = short and medium
+ * tasks) and the number of executes timed. */
 int radik_plan_set_timing(radik_plan *plan, int enable);
 int radik_plan_timing(radik_plan *plan, double *ms8, int64_t *count);
 /* Temperature / top-p (nucleus) sampling from the sorted top-k of each task
