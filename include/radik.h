@@ -160,7 +160,10 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * done; -1, the default: 0 for eager launches, 2 for a graph), "join_early"
  * (1: the plan's stream waits for the short and medium tasks before the
  * finishing kernel, 0: at the end of the launch; -1, the default: 1 when the
- * long tasks hold at least as many elements as the others), "sample_min_len" (shortest task that is
+ * long tasks hold at least as many elements as the others), "longest_first"
+ * (1, the default: tasks are staged longest first, so the one-block-per-task
+ * kernels start their slowest blocks first; 0: in the caller's order),
+ * "sample_min_len" (shortest task that is
  * sampled and gets a spill bound, default 32768; shorter ones run in
  * k_medium) and
  * "medium_per_cu"

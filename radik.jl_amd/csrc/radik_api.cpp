@@ -446,6 +446,9 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
   } else if (n == "fork_late") {
     if (value < -1 || value > 2) return set_error(RADIK_EINVAL, "fork_late must be -1, 0, 1 or 2");
     p->opts.fork_late = (int)value;
+  } else if (n == "longest_first") {
+    if (value < 0 || value > 1) return set_error(RADIK_EINVAL, "longest_first must be 0 or 1");
+    p->opts.longest_first = value != 0;
   } else if (n == "join_early") {
     if (value < -1 || value > 1) return set_error(RADIK_EINVAL, "join_early must be -1, 0 or 1");
     p->opts.join_early = (int)value;
@@ -477,6 +480,7 @@ int radik_plan_set_option(radik_plan* p, const char* name, int64_t value) {
     p->opts.medium = keep.medium;
     p->opts.fork_late = keep.fork_late;
     p->opts.join_early = keep.join_early;
+    p->opts.longest_first = keep.longest_first;
     p->opts.med_per_cu = keep.med_per_cu;
     p->opts.sample_min_len = keep.sample_min_len;
   } else {
@@ -678,6 +682,7 @@ int radik_diag_pass1(const float* val_in, const int64_t* task_lens, int64_t num_
   LaunchOpts o;
   o.predict = false;
   o.medium = false;
+  o.longest_first = false;  // (the outputs are per staged task, in the caller's order)
   rc = stage_tasks(pipe, specs.data(), num_tasks, all, o, s);
   if (rc) return rc;
   rc = upload(pipe, s);

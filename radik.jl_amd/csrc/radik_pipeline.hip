@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <mutex>
+#include <numeric>
+#include <vector>
 
 #include "../../include/radik.h"
 #include "radik_host.h"
@@ -395,10 +397,18 @@ int stage_tasks(Pipeline& p, const TaskSpec* tasks, int64_t n, const DevFacts& f
   int64_t max_m_med = 0;
   groups = 0;
   bool big = false, pos_bounds = false;
+  // Tasks are staged longest first (o.longest_first): the one-block-per-task
+  // kernels (prediction, finishing, medium tasks) then start their slowest
+  // blocks first instead of whenever a slot frees up.  Results go through
+  // each task's own pointers, so the order is free.
+  std::vector<int64_t> ord((size_t)n);
+  std::iota(ord.begin(), ord.end(), (int64_t)0);
+  if (o.longest_first)
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return tasks[a].len > tasks[b].len; });
   // pass 0: predicted long tasks; pass 1: the others
   for (int pass = 0; pass < 2; ++pass)
-  for (int64_t t = 0; t < n; ++t) {
-    const TaskSpec& ts = tasks[t];
+  for (int64_t q = 0; q < n; ++q) {
+    const TaskSpec& ts = tasks[ord[(size_t)q]];
     const bool shrt = is_short(ts.len, scap);
     const bool pred = o.predict && !shrt && sample_groups(ts.len, ts.k, o.sample_min_len) > 0;
     const bool med = !shrt && is_medium(ts.len, ts.k, pred, o);

@@ -109,6 +109,7 @@ def test_mixed_plan_launch_options():
     k = 128
     ev, ei = O.topk_exact(x, lens, k, O.LARGEST | O.ASCEND)
     sets = [{"join_early": 0}, {"join_early": 1}, {"fork_late": 1}, {"fork_late": 2, "join_early": 0},
+            {"longest_first": 0}, {"longest_first": 1},
             {"medium_per_cu": 1}, {"medium_per_cu": 8}, {"sample_min_len": 4096}, {"sample_min_len": 65536}]
     for opts in sets:
         for use_graph in (False, True):
@@ -125,6 +126,7 @@ def test_mixed_plan_launch_options():
                 same(vo.to_host(), io.to_host(), ev.T, ei.T, f"options {opts} graph={use_graph} rep{rep}")
     p = radik.Plan(lens, k, largest=True, rev=False)
     for name, bad in (("join_early", 2), ("fork_late", 3), ("medium_per_cu", 0), ("sample_min_len", 100),
+                      ("longest_first", 2),
                       ("no_such_option", 1)):
         with pytest.raises(radik.RadiKError):
             p.set_option(name, bad)
