@@ -157,7 +157,7 @@ int radik_plan_set_options(radik_plan *plan, int vload_bytes, int blocks_per_cu,
  * the streaming passes and are re-read by the finishing kernel), and
  * "fork_late" (0/1/2: the short and medium tasks' kernels start with the
  * launch, once the long tasks' prediction is done, or once their sampling is
- * done; -1, the default: 0 for eager launches, 2 for a graph), "join_early"
+ * done; -1, the default: 2), "join_early"
  * (1: the plan's stream waits for the short and medium tasks before the
  * finishing kernel, 0: at the end of the launch; -1, the default: 1 when the
  * long tasks hold at least as many elements as the others), "longest_first"

@@ -57,7 +57,7 @@ struct LaunchOpts {
   bool predict = true;      // sample long tasks and spill in pass A
   bool medium = true;       // medium tasks to k_medium (else the streaming passes)
   int fork_late = -1;       // short/medium tasks start with the launch (0), after the prediction (1)
-                            // or after the sampling (2); -1: 0 for eager launches, 2 for a graph
+                            // or after the sampling (2); -1: 2
   bool longest_first = true;  // tasks staged longest first (else in the caller's order)
   int join_early = -1;      // the launch's stream waits for the short/medium tasks before k_finish (1)
                             // or at the end (0); -1: 1 when the long tasks hold at least as many

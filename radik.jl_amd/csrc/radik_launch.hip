@@ -98,12 +98,12 @@ static int enqueue_all(Pipeline& p, const DevFacts& f, const LaunchOpts& o, hipS
     RADIK_HIP(hipEventRecord(p.ev_join, p.side));
     return 0;
   };
-  // fork_late: the side stream starts once the prediction is done (1), so
-  // the sampling and prediction have the whole chip and the short and medium
-  // tasks overlap pass A only, or once the sampling is done (2).  By default
-  // eager launches fork at once and a captured graph after the sampling (its
-  // cross-stream edges cost more; measured best for each).
-  const int fl = o.fork_late >= 0 ? o.fork_late : (o.graph ? 2 : 0);
+  // fork_late: the side stream starts with the launch (0), once the
+  // prediction is done (1: the sampling and prediction have the whole chip
+  // and the short and medium tasks overlap pass A only) or once the sampling
+  // is done (2, the default: measured best eagerly and as a graph with the
+  // tasks staged longest first).
+  const int fl = o.fork_late >= 0 ? o.fork_late : 2;
   if (fork && fl == 0 && (rc = fork_side())) return rc;
   const bool join_here =
       o.join_early >= 0 ? o.join_early != 0 : p.total >= p.total_med + p.total_small;  // (before k_finish)
